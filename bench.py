@@ -1,0 +1,206 @@
+"""Benchmark: trimem energy + gradient evaluations per second on one MI355X.
+
+Workload (the single-mesh precision configuration): icosphere level 7
+(163842 vertices, 327680 faces), radius 1, vertices jittered by
+N(0, (0.01 * mean edge)^2) from seed 20261203; kappa_b=30, kappa_a=1e6,
+kappa_v=1e6, kappa_c=0, kappa_t=1e5, kappa_r=1e3, area fraction 1.0, volume
+fraction 0.8, Edge tether (r=2, lc0=1.25 l, lc1=0.75 l), cell-list repulsion
+with exclusion level 2 and rc = rlist = 1.5 l.  One step = one energy + gradient
+evaluation (properties pass + gradient pass) on positions already resident in
+HBM, in the mixed precision path by default.
+
+Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC_FULL = "energy+grad evals/s & HBM GB/s; surrogate vs plain HMC effective samples/s"
+HBM_PEAK_GBPS = 8000.0
+
+
+def workload(level):
+    from trimem_amd import meshes
+    v, f, lbar = meshes.jittered_icosphere(level, seed=20261203, rel_sigma=0.01)
+    params = dict(kappa_b=30.0, kappa_a=1e6, kappa_v=1e6, kappa_c=0.0, kappa_t=1e5, kappa_r=1e3,
+                  area_frac=1.0, volume_frac=0.8, curvature_frac=1.0,
+                  bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, bond_a0=1.0, bond_r=2, bond_type=0,
+                  rep_lc1=1.5 * lbar, rep_r=2, rep_nlist=0, rep_rlist=1.5 * lbar, rep_exclusion=2,
+                  cont_delta=0.0, cont_lambda=1.0)
+    return v, f, lbar, params
+
+
+class Hip:
+    """hipEvent timing on the library's stream."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        t = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
+        return float(t.value)
+
+
+def cpu_baseline(v, f, params, budget_s):
+    """The FP64 OpenMP oracle (the reference's vertex-centric structure) on
+    the same inputs: one eval = energy() + gradient() as the reference's
+    callers issue them (2 properties passes + 1 gradient pass)."""
+    from oracle import oracle as O
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, O.Params(**params))
+    for _ in range(2):
+        oe.energy(om)
+        oe.gradient(om)
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < 3 or (time.perf_counter() < t_end and len(times) < 50):
+        t0 = time.perf_counter()
+        oe.energy(om)
+        oe.gradient(om)
+        times.append(time.perf_counter() - t0)
+    return 1.0 / float(np.median(times)), len(times), O.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: every box here has a single MI355X; "
+              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from trimem_amd import capi
+    L = capi.lib()
+    v, f, lbar, params = workload(args.level)
+    n, nf = len(v), len(f)
+
+    mesh = ctypes.c_void_p()
+    vv = np.ascontiguousarray(v)
+    ff = np.ascontiguousarray(f, dtype=np.int32)
+    capi.check(L.tm_mesh_create(vv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                ff.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), nf, ctypes.byref(mesh)))
+    cp = capi.tm_params()
+    for k, _ in capi.tm_params._fields_:
+        if k != "reserved":
+            setattr(cp, k, params[k])
+    em = ctypes.c_void_p()
+    capi.check(L.tm_energy_create(mesh, ctypes.byref(cp), ctypes.byref(em)))
+    capi.check(L.tm_energy_set_precision(em, capi.TM_PREC_MIXED if args.precision == "mixed" else capi.TM_PREC_FP64))
+    capi.check(L.tm_mesh_upload_points(mesh))
+    stream = ctypes.c_void_p()
+    capi.check(L.tm_stream(ctypes.byref(stream)))
+    hip = Hip()
+    both = capi.TM_EVAL_PROPS | capi.TM_EVAL_GRAD
+
+    for _ in range(args.warmup):
+        capi.check(L.tm_energy_eval_device(em, mesh, both))
+    capi.check(L.tm_sync())
+
+    # timed region: K evaluations end to end
+    e0, e1 = hip.event(), hip.event()
+    capi.check(L.tm_sync())
+    t0 = time.perf_counter()
+    hip.record(e0, stream)
+    for _ in range(args.steps):
+        L.tm_energy_eval_device(em, mesh, both)
+    hip.record(e1, stream)
+    capi.check(L.tm_sync())
+    wall = time.perf_counter() - t0
+    ms_total = hip.elapsed_ms(e0, e1)
+    evals_per_s = args.steps / (ms_total / 1e3)
+
+    # per-kernel durations, events around each pass on the same stream
+    ks = min(args.steps, 500)
+    evs = [[hip.event() for _ in range(3)] for _ in range(ks)]
+    for i in range(ks):
+        hip.record(evs[i][0], stream)
+        L.tm_energy_eval_device(em, mesh, capi.TM_EVAL_PROPS)
+        hip.record(evs[i][1], stream)
+        L.tm_energy_eval_device(em, mesh, capi.TM_EVAL_GRAD)
+        hip.record(evs[i][2], stream)
+    capi.check(L.tm_sync())
+    p1 = np.array([hip.elapsed_ms(e[0], e[1]) for e in evs])
+    p2 = np.array([hip.elapsed_ms(e[1], e[2]) for e in evs])
+    capi.check(L.tm_energy_eval_device(em, mesh, both))
+    capi.check(L.tm_sync())
+
+    # algorithmic bytes per vertex: pass 1 reads positions (24),
+    # ring ids (4 per neighbour, 2E/N = 6 on a closed mesh), meta (4) and
+    # writes q (8); pass 2 reads positions, ring ids, meta and q of the ring
+    # and writes the gradient (24).
+    ring_b = 4.0 * 2 * (3 * n - 6) / n if n > 0 else 0.0
+    b_pass1 = n * (24 + ring_b + 4 + 8)
+    b_pass2 = n * (24 + ring_b + 4 + 8 + 24)
+    t1, t2 = float(np.mean(p1)) * 1e-3, float(np.mean(p2)) * 1e-3
+    dom = ("k_pass2", b_pass2, t2) if t2 >= t1 else ("k_pass1", b_pass1, t1)
+    achieved = dom[1] / dom[2] / 1e9
+    eval_bytes = b_pass1 + b_pass2
+
+    out = {
+        "metric": "energy+grad evals/s",
+        "metric_full": METRIC_FULL,
+        "value": evals_per_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosphere L{args.level} ({n} vertices, {nf} faces), jittered 1% of mean edge, "
+                               "kappa_b=30 kappa_a=1e6 kappa_v=1e6 kappa_t=1e5 kappa_r=1e3, vfrac 0.8, "
+                               "edge tether, cell-list repulsion excl 2 rc=1.5l; one step = energy+gradient",
+                   "precision": args.precision, "vertices": n, "faces": nf},
+        "hbm_gbps_per_eval": eval_bytes * evals_per_s / 1e9,
+        "kernels_ms": {"k_pass1": float(np.mean(p1)), "k_pass2": float(np.mean(p2))},
+        "roofline": {"kernel": dom[0], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes": dom[1]},
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu_baseline:
+        cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",
+                               "sample": f"{ns} FP64 energy()+gradient() evals of the same L{args.level} mesh, "
+                                         f"median, OpenMP {thr} threads, {platform.processor() or 'host'}"}
+        out["speedup_vs_cpu"] = evals_per_s / cv
+    print(json.dumps(out), flush=True)
+    L.tm_energy_destroy(em)
+    L.tm_mesh_destroy(mesh)
+
+
+if __name__ == "__main__":
+    main()

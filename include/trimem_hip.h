@@ -1,0 +1,167 @@
+/* trimem_hip.h -- C ABI of the MI355X (gfx950) trimem energy / gradient /
+ * neighbour-list / flip library (libtrimem_hip.so).
+ *
+ * Drop-in boundary for bio-phys/trimem's compiled core: every entry point
+ * below replaces one C++ entry of src/ that src/bindings.cpp exposes as
+ * trimem.core (bindings.cpp:1038-1093); the citation after each declaration
+ * names it.  Plain pointers and sizes only; no exception crosses the ABI.
+ *
+ * Conventions
+ *   - Every function returns TM_OK (0) or a negative TM_E* code; the message
+ *     of the last failure on the calling thread is tm_last_error().
+ *   - Host arrays are row-major and caller-owned: points double[n*3],
+ *     faces int[m*3], gradients double[n*3], exactly numpy's (n,3) C order.
+ *   - A tm_mesh owns a host points buffer (tm_mesh_points) that the caller
+ *     may write at any time, like trimem's TriMesh.points() view
+ *     (mesh_py.cpp:96-105); every host-facing compute call uploads it first.
+ *   - The *_device entry points work on the device-resident positions
+ *     (tm_mesh_device_points) and leave results in device memory; they do
+ *     not synchronise.
+ *   - Handles are not thread-safe: one thread at a time per handle, which is
+ *     how the reference is used under the Python GIL.
+ */
+#ifndef TRIMEM_HIP_H
+#define TRIMEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TM_OK = 0,
+    TM_E_INVALID = -1,   /* bad argument or shape (reference: std::runtime_error)  */
+    TM_E_RANGE = -2,     /* flip_ratio > 1 for the parallel flips (std::range_error) */
+    TM_E_DEVICE = -3,    /* HIP runtime failure or no device                        */
+    TM_E_UNSUPPORTED = -4
+};
+
+typedef struct tm_mesh tm_mesh;
+typedef struct tm_energy tm_energy;
+typedef struct tm_nlist tm_nlist;
+
+/* Flattened EnergyParams / BondParams / SurfaceRepulsionParams /
+ * ContinuationParams (src/params.h:11-71). */
+typedef struct {
+    double kappa_b, kappa_a, kappa_v, kappa_c, kappa_t, kappa_r;
+    double area_frac, volume_frac, curvature_frac;
+    double bond_lc0, bond_lc1, bond_a0;
+    int bond_r;
+    int bond_type;      /* 0 Edge, 1 Area, 2 None (params.h:13-18)            */
+    double rep_lc1;
+    int rep_r;
+    int rep_nlist;      /* 0 "cell-list", 1 "verlet-list" (nlist.cpp:14-50) */
+    double rep_rlist;
+    int rep_exclusion;  /* 0, 1 or 2                                         */
+    int reserved;
+    double cont_delta, cont_lambda;
+} tm_params;
+
+/* TVertexProperties<real> (mesh_properties.h:15-48) */
+typedef struct {
+    double area, volume, curvature, bending, tethering, repulsion;
+} tm_props;
+
+enum { TM_PREC_FP64 = 0, TM_PREC_MIXED = 1 };
+
+const char* tm_last_error(void);
+int tm_device_count(int* count);
+int tm_set_device(int device);
+
+/* ---- mesh: TriMesh (mesh.h:15-31, mesh_py.cpp:12-105, bindings.cpp:52-193) ---- */
+/* TriMesh(points, faces) -> from_points_cells (mesh_py.cpp:12-69) */
+int tm_mesh_create(const double* points, int n_points, const int* faces, int n_faces, tm_mesh** out);
+int tm_mesh_destroy(tm_mesh* mesh);
+/* n_vertices / n_edges / n_faces (bindings.cpp:136-146) */
+int tm_mesh_counts(const tm_mesh* mesh, int* n_vertices, int* n_edges, int* n_faces);
+/* points() view (mesh_py.cpp:96-105): host buffer owned by the mesh */
+double* tm_mesh_points(tm_mesh* mesh);
+/* fv_indices (mesh_py.cpp:73-93) */
+int tm_mesh_fv_indices(tm_mesh* mesh, int* out);
+/* halfedge_handle(i) / next_halfedge_handle / opposite_halfedge_handle
+ * (bindings.cpp:148-174); opposite is h ^ 1 */
+int tm_mesh_next_halfedge(tm_mesh* mesh, int he, int* out);
+/* halfedge arrays (to, next, face per halfedge; halfedge per vertex/face) */
+int tm_mesh_topology(tm_mesh* mesh, int* he_to, int* he_next, int* he_face, int* v_he, int* f_he);
+/* device-resident positions: double[n*3] in HBM */
+double* tm_mesh_device_points(tm_mesh* mesh);
+int tm_mesh_upload_points(tm_mesh* mesh);
+int tm_mesh_download_points(tm_mesh* mesh);
+
+/* ---- geometry on one halfedge (mesh_util.h:15-185, bindings.cpp:244-407) ----
+ * out[47] = [edge_length, edge_length_grad(2x3), face_area, face_area_grad(3x3),
+ *            face_volume, face_volume_grad(3x3), dihedral_angle,
+ *            dihedral_angle_grad(4x3)] */
+int tm_halfedge_geometry(tm_mesh* mesh, int he, double* out47);
+
+/* ---- utilities (util.cpp:9-41, bindings.cpp:987-1036) ---- */
+int tm_area(tm_mesh* mesh, double* out);
+int tm_edges_length(tm_mesh* mesh, double* out);
+
+/* ---- energy: EnergyManager (energy.h:18-56, energy.cpp:13-151) ---- */
+/* EnergyManager(mesh, eparams) (energy.cpp:13-28); bond type None and
+ * unknown list types or exclusion levels fail with TM_E_INVALID
+ * (mesh_tether.h:123-135, nlist.cpp:10-53) */
+int tm_energy_create(tm_mesh* mesh, const tm_params* params, tm_energy** out);
+int tm_energy_destroy(tm_energy* em);
+int tm_energy_get_params(const tm_energy* em, tm_params* out);
+int tm_energy_initial_props(const tm_energy* em, tm_props* out);
+/* precision of properties/energy/gradient: TM_PREC_FP64 (default) or mixed */
+int tm_energy_set_precision(tm_energy* em, int precision);
+/* properties(mesh) -> reduced VertexProperties (energy.cpp:63-77) */
+int tm_energy_properties(tm_energy* em, tm_mesh* mesh, tm_props* out);
+/* per-vertex VertexProperties vector (EvaluateProperties, kernel.h:140-169) */
+int tm_energy_vertex_properties(tm_energy* em, tm_mesh* mesh, double* out /* n*6 */);
+/* energy(mesh) (energy.cpp:79-85) */
+int tm_energy_energy(tm_energy* em, tm_mesh* mesh, double* out);
+/* energy(vprops) (energy.cpp:87-91) */
+int tm_energy_energy_props(tm_energy* em, const tm_props* props, double* out);
+/* gradient(mesh) (energy.cpp:93-124) */
+int tm_energy_gradient(tm_energy* em, tm_mesh* mesh, double* out /* n*3 */);
+/* energy(mesh) and gradient(mesh) from one properties pass */
+int tm_energy_energy_gradient(tm_energy* em, tm_mesh* mesh, double* energy, double* grad /* n*3 */);
+/* update_reference_properties (energy.cpp:47-55) */
+int tm_energy_update_reference_properties(tm_energy* em);
+/* update_repulsion (energy.cpp:57-61) */
+int tm_energy_update_repulsion(tm_energy* em, tm_mesh* mesh);
+/* device-resident evaluation on tm_mesh_device_points (no counterpart in the
+ * reference, whose gradient() always starts from host points): flags bit 0
+ * runs the properties pass (energy, globals), bit 1 the gradient pass (which
+ * needs a properties pass before it).  Enqueued on the library stream
+ * (tm_stream), no synchronisation; results via the pointers below. */
+enum { TM_EVAL_PROPS = 1, TM_EVAL_GRAD = 2 };
+int tm_energy_eval_device(tm_energy* em, tm_mesh* mesh, int flags);
+const double* tm_energy_device_gradient(tm_energy* em);   /* double[n*3] */
+const double* tm_energy_device_globals(tm_energy* em);    /* props[6], energy */
+int tm_sync(void);
+/* the library's HIP stream (hipStream_t), for event timing by callers */
+int tm_stream(void** out);
+
+/* ---- neighbour lists (nlist.h:15-24, nlist.cpp:10-53, bindings.cpp:893-985) ---- */
+/* make_nlist(mesh, eparams) */
+int tm_nlist_create(tm_mesh* mesh, const tm_params* params, tm_nlist** out);
+int tm_nlist_destroy(tm_nlist* nl);
+/* the energy manager's own list (EnergyManager::nlist), borrowed */
+int tm_energy_nlist(tm_energy* em, tm_nlist** out);
+/* distance_matrix(mesh, dmax): count first (d, i, j may be NULL), then fill */
+int tm_nlist_distance_matrix(tm_nlist* nl, tm_mesh* mesh, double dmax, double* d, int* i, int* j, long cap,
+                             long* count);
+/* point_distances(mesh, pid, dmax) */
+int tm_nlist_point_distances(tm_nlist* nl, tm_mesh* mesh, int pid, double dmax, double* dist3, int* jdx,
+                             long cap, long* count);
+
+/* ---- flips (flips.h:11-17, bindings.cpp:839-891) ---- */
+/* flip(mesh, estore, flip_ratio) with the random stream drawn from seed
+ * (the reference draws from a clock-seeded mt19937, flips.cpp:20) */
+int tm_flip_serial(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int* accepted);
+/* flip_serial with caller-supplied proposals (edge ids) and uniforms; a
+ * uniform is consumed only by proposals that pass is_flip_ok */
+int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const int* proposals,
+                            const double* uniforms, int n_uniforms, int* accepted, int* n_used);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,296 @@
+"""GPU parity: trimem_amd.core (HIP kernels via the C ABI) vs the FP64 oracle.
+
+Tolerances (stated per check):
+  * FP64 path: global properties and energy rel 1e-10, gradient
+    ||g - g_ref|| / ||g_ref|| <= 1e-10 (the two differ only in summation
+    order and in the regrouped gradient formula);
+  * mixed path: gradient rel <= 1e-6, energy rel <= 1e-9;
+  * neighbour lists: pair sets and distances bit-exact;
+  * serial flips with injected streams: topology (fv_indices) bit-exact,
+    same accepted and consumed-uniform counts.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trimem_amd import meshes as M
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+
+
+@pytest.fixture(scope="module")
+def core():
+    import trimem_amd.core as m
+    assert m.device_count() > 0, "no GPU visible"
+    return m
+
+
+def to_core_params(m, p):
+    e = m.EnergyParams()
+    for k in ("kappa_b", "kappa_a", "kappa_v", "kappa_c", "kappa_t", "kappa_r",
+              "area_frac", "volume_frac", "curvature_frac"):
+        setattr(e, k, getattr(p, k))
+    b = m.BondParams()
+    b.lc0, b.lc1, b.a0, b.r = p.bond_lc0, p.bond_lc1, p.bond_a0, p.bond_r
+    b.type = [m.BondType.Edge, m.BondType.Area, getattr(m.BondType, "None")][p.bond_type]
+    e.bond_params = b
+    r = m.SurfaceRepulsionParams()
+    r.lc1, r.r, r.rlist, r.exclusion_level = p.rep_lc1, p.rep_r, p.rep_rlist, p.rep_exclusion
+    r.n_search = ["cell-list", "verlet-list"][p.rep_nlist]
+    e.repulse_params = r
+    c = m.ContinuationParams()
+    c.delta, c.lam = p.cont_delta, p.cont_lambda
+    e.continuation_params = c
+    return e
+
+
+def props_vec(vp):
+    return np.array([vp.area, vp.volume, vp.curvature, vp.bending, vp.tethering, vp.repulsion])
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def base_params(lbar, **kw):
+    d = dict(kappa_b=30.0, kappa_a=1e6, kappa_v=1e6, kappa_c=1e3, kappa_t=1e5, kappa_r=1e3,
+             area_frac=0.95, volume_frac=0.8, curvature_frac=1.05, bond_type=0,
+             bond_lc0=1.02 * lbar, bond_lc1=0.98 * lbar, bond_a0=1.0, bond_r=2)
+    d.update(kw)
+    return O.Params(**d)
+
+
+def make_pair(core, v, f, p):
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, p)
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    return om, oe, gm, ge
+
+
+def check_fp64(core, v, f, p, tol=1e-10):
+    om, oe, gm, ge = make_pair(core, v, f, p)
+    assert rel(props_vec(ge.initial_props), oe.initial_props()) < tol
+    po = oe.properties(om)
+    pg = props_vec(ge.properties(gm))
+    for k in range(6):
+        assert abs(pg[k] - po[k]) <= tol * max(abs(po[k]), 1e-12), (k, pg[k], po[k])
+    eo, eg = oe.energy(om), ge.energy(gm)
+    assert abs(eg - eo) <= tol * abs(eo)
+    go, gg = oe.gradient(om), ge.gradient(gm)
+    assert rel(gg, go) <= tol, rel(gg, go)
+    return om, oe, gm, ge
+
+
+# ---------------------------------------------------------- primitives -----
+@pytest.mark.parametrize("alpha", [np.pi / 4, -np.pi / 4])
+def test_halfedge_primitives(core, alpha):
+    x, z = np.cos(alpha) * 0.5, np.sin(alpha) * 0.5
+    pts = np.array([[0.0, -0.5, 0.0], [0.0, 0.5, 0.0], [-x, 0.0, z], [x, 0.0, z]])
+    cells = np.array([[0, 1, 2], [1, 0, 3]])
+    om, gm = O.Mesh(pts, cells), core.TriMesh(pts, cells)
+    for h in range(2 * gm.n_edges()):
+        he = gm.halfedge_handle(h)
+        assert abs(core.edge_length(gm, he) - om.edge_length(h)) < 1e-14
+        assert rel(core.edge_length_grad(gm, he), om.edge_length_grad(h)) < 1e-14
+        if gm.n_edges() and h in (0, 1):
+            assert abs(core.face_area(gm, he) - om.face_area(h)) < 1e-14
+            assert rel(core.face_area_grad(gm, he), om.face_area_grad(h)) < 1e-13
+            assert abs(core.face_volume(gm, he) - om.face_volume(h)) < 1e-14
+            assert rel(core.face_volume_grad(gm, he), om.face_volume_grad(h)) < 1e-13
+        assert abs(core.dihedral_angle(gm, he) - om.dihedral_angle(h)) < 1e-13
+    he = gm.halfedge_handle(0)
+    assert rel(core.dihedral_angle_grad(gm, he), om.dihedral_angle_grad(0)) < 1e-12
+
+
+# ------------------------------------------------------- energy, FP64 ------
+@pytest.mark.parametrize("bond", [0, 1])
+def test_fp64_icosphere(core, bond):
+    v, f, lbar = M.jittered_icosphere(3, seed=20261202, rel_sigma=0.05)
+    a = 4 * np.pi / len(f)
+    check_fp64(core, v, f, base_params(lbar, bond_type=bond, bond_a0=a))
+
+
+@pytest.mark.parametrize("nlist,excl", [(0, 0), (0, 1), (0, 2), (1, 0), (1, 2)])
+def test_fp64_repulsion(core, nlist, excl):
+    v, f, lbar = M.jittered_icosphere(3, seed=3, rel_sigma=0.02)
+    v = v * np.array([1.0, 1.0, 0.06])
+    lc1 = 2.0 * lbar
+    p = base_params(lbar, rep_lc1=lc1, rep_rlist=lc1, rep_nlist=nlist, rep_exclusion=excl, kappa_r=1.0,
+                    kappa_a=1e2, kappa_v=1e2, kappa_t=1.0)
+    om, oe, gm, ge = check_fp64(core, v, f, p)
+    assert oe.properties(om)[5] > 0
+
+
+def test_fp64_open_tube(core):
+    """Boundary vertices take the OpenMesh-semantics walker (reference
+    boundary branches included)."""
+    v, f = M.tube(length=1.0, radius=0.5, segments=24)
+    lbar = M.mean_edge_length(v, f)
+    rng = np.random.default_rng(4)
+    v = v + rng.normal(0, 0.02 * lbar, v.shape)
+    check_fp64(core, v, f, base_params(lbar, kappa_a=10.0, kappa_v=10.0, kappa_t=1.0, volume_frac=0.9))
+
+
+def test_doc_energy_value(core):
+    v, f = M.icosphere(3)
+    v = v.astype(np.float32).astype(np.float64)
+    gm = core.TriMesh(v, f)
+    p = O.Params(**GOLD["doc_energy"]["params"])
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    ref = GOLD["doc_energy"]["energy"]
+    assert abs(ge.energy(gm) - ref) / ref < 1e-12
+    g = ge.gradient(gm)
+    for vid, r in zip((0, 532, 196), GOLD["doc_energy"]["gradient_rows_first"]):
+        assert np.linalg.norm(g[vid] - r) / np.linalg.norm(r) < 5e-9
+
+
+def test_points_view_and_continuation(core):
+    v, f, lbar = M.jittered_icosphere(3, seed=8, rel_sigma=0.05)
+    p = base_params(lbar, cont_delta=0.25, cont_lambda=0.0)
+    om, oe, gm, ge = make_pair(core, v, f, p)
+    x = gm.points()
+    x += 0.01 * np.sin(7 * x)
+    om.x = x
+    for _ in range(3):
+        ge.update_reference_properties()
+        oe.update_reference_properties()
+    assert abs(ge.eparams.continuation_params.lam - 0.75) < 1e-15
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+
+
+def test_gradient_deterministic(core):
+    v, f, lbar = M.jittered_icosphere(4, seed=9)
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, base_params(lbar)))
+    g1, g2 = ge.gradient(gm), ge.gradient(gm)
+    assert np.array_equal(g1, g2)
+
+
+# ------------------------------------------------------------ mixed --------
+@pytest.mark.parametrize("level", [4, 5])
+def test_mixed_precision(core, level):
+    v, f, lbar = M.jittered_icosphere(level, seed=20261203)
+    p = base_params(lbar, kappa_c=0.0, area_frac=1.0, volume_frac=0.8, curvature_frac=1.0,
+                    bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar)
+    om, oe, gm, ge = make_pair(core, v, f, p)
+    ge.precision = "mixed"
+    go = oe.gradient(om)
+    gg = ge.gradient(gm)
+    err = rel(gg, go)
+    print(f"mixed gradient rel err L{level}: {err:.3e}")
+    assert err <= 1e-6
+    eo = oe.energy(om)
+    assert abs(ge.energy(gm) - eo) <= 1e-9 * abs(eo)
+
+
+# ----------------------------------------------------- neighbour lists -----
+@pytest.mark.parametrize("ltype", [0, 1])
+@pytest.mark.parametrize("excl", [0, 1, 2])
+def test_exclusion_counts(core, ltype, excl):
+    g = GOLD["exclusion_counts"]
+    v, f = M.rect(20, 20)
+    gm = core.TriMesh(v, f)
+    p = O.Params(rep_nlist=ltype, rep_rlist=g["rlist"], rep_exclusion=excl)
+    nl = core.make_nlist(gm, to_core_params(core, p))
+    _, jdx = nl.point_distances(gm, g["pid"], g["rlist"])
+    assert len(jdx) == g["counts"][str(excl)]
+
+
+@pytest.mark.parametrize("ltype", [0, 1])
+@pytest.mark.parametrize("excl", [0, 2])
+def test_nlist_bit_exact(core, ltype, excl):
+    v, f, lbar = M.jittered_icosphere(3, seed=5, rel_sigma=0.02)
+    v = v * np.array([1.0, 0.8, 0.07])
+    rl = 1.5 * lbar
+    p = O.Params(rep_nlist=ltype, rep_rlist=rl, rep_exclusion=excl)
+    om, gm = O.Mesh(v, f), core.TriMesh(v, f)
+    onl = O.NeighbourList(om, p)
+    gnl = core.make_nlist(gm, to_core_params(core, p))
+    for dmax in (rl, 0.7 * rl):
+        do, io, jo = onl.distance_matrix(om, dmax)
+        dg, ig, jg = gnl.distance_matrix(gm, dmax)
+        a = sorted(zip(io.tolist(), jo.tolist(), do.tolist()))
+        b = sorted(zip(ig, jg, dg))
+        assert a == b
+    for pid in range(0, len(v), 3):
+        d0, j0 = onl.point_distances(om, pid, rl)
+        d1, j1 = gnl.point_distances(gm, pid, rl)
+        assert list(j0) == list(j1) if ltype == 0 else sorted(j0) == sorted(j1)
+        d1 = np.array([[q.x, q.y, q.z] for q in d1]).reshape((-1, 3))
+        if ltype == 0:
+            assert np.array_equal(d0, d1)
+
+
+def test_nlist_after_motion(core):
+    """Cells stay frozen between rebuilds; queries use current positions."""
+    v, f, lbar = M.jittered_icosphere(3, seed=6, rel_sigma=0.02)
+    v = v * np.array([1.0, 1.0, 0.08])
+    rl = 2.0 * lbar
+    p = base_params(lbar, rep_lc1=rl, rep_rlist=rl, rep_nlist=0, rep_exclusion=2, kappa_r=1.0,
+                    kappa_a=1e2, kappa_v=1e2, kappa_t=1.0)
+    om, oe, gm, ge = make_pair(core, v, f, p)
+    rng = np.random.default_rng(1)
+    x = v + rng.normal(0, 0.3 * lbar, v.shape)
+    gm.points()[:] = x
+    om.x = x
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+    ge.update_repulsion(gm)
+    oe.update_repulsion(om)
+    assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+
+
+# ---------------------------------------------------------------- utils ----
+def test_mesh_utils(core):
+    v, f, lbar = M.jittered_icosphere(4, seed=2)
+    om, gm = O.Mesh(v, f), core.TriMesh(v, f)
+    assert abs(core.area(gm) - om.area()) < 1e-12 * om.area()
+    assert abs(core.edges_length(gm) - om.edges_length()) < 1e-12 * om.edges_length()
+    a, l = core.avg_tri_props(gm)
+    ao, lo = om.avg_tri_props()
+    assert abs(a - ao) < 1e-12 * ao and abs(l - lo) < 1e-12 * lo
+
+
+# ---------------------------------------------------------------- flips ----
+def flip_case(level, seed, rep):
+    v, f, lbar = M.jittered_icosphere(level, seed=seed, rel_sigma=0.05)
+    kw = dict(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0, kappa_r=1.0,
+              area_frac=1.0, volume_frac=1.0, curvature_frac=1.0, bond_lc0=1.15 * lbar, bond_lc1=0.85 * lbar)
+    if rep:
+        kw.update(rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar, rep_exclusion=1)
+    return v, f, O.Params(**kw)
+
+
+@pytest.mark.parametrize("level,rep", [(2, False), (3, False), (3, True)])
+def test_flip_serial_injected_parity(core, level, rep):
+    v, f, p = flip_case(level, 11 + level, rep)
+    om, oe, gm, ge = make_pair(core, v, f, p)
+    rng = np.random.default_rng(level)
+    n = int(gm.n_edges() * 0.5)
+    prop = rng.integers(0, gm.n_edges(), n).astype(np.int32)
+    uni = rng.random(n)
+    acc_o, used_o = oe.flip_serial_injected(om, prop, uni)
+    acc_g, used_g = core.flip_injected(gm, ge, prop, uni)
+    assert (acc_g, used_g) == (acc_o, used_o)
+    assert np.array_equal(gm.fv_indices(), om.fv_indices())
+    # energy and gradient agree on the flipped topology
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+
+
+def test_flip_smoke_reference_bound(core):
+    """tests/test_flips.py:52-60: > 100 accepted at flip_ratio 0.5."""
+    v, f = M.icosphere(3)
+    gm = core.TriMesh(v, f)
+    a, l = core.avg_tri_props(gm)
+    p = O.Params(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0,
+                 bond_lc0=1.15 * l, bond_lc1=0.85 * l, bond_a0=a)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    assert core.flip(gm, ge, 0.5, seed=1) > 100

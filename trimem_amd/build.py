@@ -1,0 +1,75 @@
+"""Build libtrimem_hip.so (gfx950 HIP kernels + C ABI) and the pybind11
+module trimem_amd.core in-tree.
+
+    python -m trimem_amd.build          # or __graft_entry__.build()
+
+hipcc compiles every .hip file for gfx950 only.  -ffp-contract=off keeps the
+neighbour-list distance tests and the FP64 path IEEE-exact (no fused a*b+c),
+and packed FP32 instructions are disabled.
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libtrimem_hip.so")
+
+HIP_SOURCES = ["tm_capi.hip", "tm_energy.hip", "tm_nlist.hip", "tm_flips.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def core_path():
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "core" + suffix)
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_lib():
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+
+
+def build_core():
+    import pybind11
+    out = core_path()
+    tmp = out + ".tmp"
+    cmd = [
+        "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+        "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+        os.path.join(CSRC, "bindings.cpp"),
+        "-L", HERE, "-ltrimem_hip", "-Wl,-rpath,$ORIGIN",
+        "-o", tmp,
+    ]
+    _run(cmd)
+    os.replace(tmp, out)
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all():
+    build_lib()
+    build_core()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "lib": build_lib, "core": build_core, "oracle": build_oracle}[w]()

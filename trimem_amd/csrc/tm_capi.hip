@@ -1,0 +1,939 @@
+// tm_capi.hip -- host side of libtrimem_hip: mesh construction, device
+// buffers and the C ABI declared in include/trimem_hip.h.
+//
+// Every compute entry point enqueues gfx950 kernels on one library stream;
+// there is no CPU compute path.  Host work is limited to building the
+// half-edge arrays from a face list (OpenMesh add_face numbering), scalar
+// parameter arithmetic, and copying inputs and results.
+#include "../../include/trimem_hip.h"
+#include "tm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace tmh;
+
+// ------------------------------------------------------------- errors -----
+namespace {
+
+thread_local std::string g_err;
+
+struct TmError : std::runtime_error {
+    int code;
+    TmError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw TmError(TM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+#define API_BEGIN try {
+#define API_END                                          \
+    }                                                    \
+    catch (const TmError& ex) {                          \
+        g_err = ex.what();                               \
+        return ex.code;                                  \
+    }                                                    \
+    catch (const std::exception& ex) {                   \
+        g_err = ex.what();                               \
+        return TM_E_INVALID;                             \
+    }                                                    \
+    return TM_OK;
+
+// ------------------------------------------------------------ runtime -----
+struct Runtime {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+Runtime g_rt;
+
+void ensure_runtime()
+{
+    if (g_rt.init) return;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw TmError(TM_E_DEVICE, "no HIP device available");
+    HIPCHECK(hipSetDevice(g_rt.device));
+    HIPCHECK(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+    g_rt.init = true;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        HIPCHECK(hipMalloc((void**)&p, bytes));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+void h2d(void* d, const void* h, size_t bytes)
+{
+    if (bytes) HIPCHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_rt.stream));
+}
+void d2h(void* h, const void* d, size_t bytes)
+{
+    if (bytes) HIPCHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, g_rt.stream));
+}
+void sync() { HIPCHECK(hipStreamSynchronize(g_rt.stream)); }
+void check_launch() { HIPCHECK(hipGetLastError()); }
+
+}    // namespace
+
+// --------------------------------------------------------------- mesh -----
+struct tm_mesh {
+    int nv = 0, nh = 0, nf = 0;
+    double* points = nullptr;    // host (N,3), pinned when a device exists
+    bool pinned = false;
+    // host mirror of the topology
+    std::vector<int> he_to, he_next, he_prev, he_face, v_he, f_he;
+    bool host_stale = false;     // device topology changed (flips)
+    bool has_boundary = false;
+    // device state
+    bool dev = false;
+    DevBuf<double> x;
+    DevBuf<int> d_to, d_next, d_prev, d_face, d_vhe, d_fhe, ring, meta, flag;
+
+    MeshView view() const
+    {
+        return MeshView{x.p, d_to.p, d_next.p, d_prev.p, d_face.p, d_vhe.p, d_fhe.p, ring.p, meta.p, nv, nh, nf};
+    }
+    FlipMeshView flip_view()
+    {
+        return FlipMeshView{x.p, d_to.p, d_next.p, d_prev.p, d_face.p, d_vhe.p, d_fhe.p, nv, nh, nf};
+    }
+    ~tm_mesh()
+    {
+        if (points) {
+            if (pinned) (void)hipHostFree(points);
+            else std::free(points);
+        }
+    }
+};
+
+namespace {
+
+// Face list -> half-edge arrays with OpenMesh's numbering
+// (PolyConnectivity::add_face as used by from_points_cells, mesh_py.cpp:53-67):
+// face ids outside [0, n) are dropped, faces with fewer than three valid ids
+// are skipped, a new edge e gets halfedge 2e = v_i -> v_{i+1}, the face's
+// halfedge is v_2 -> v_0, and faces that would create a complex vertex (one
+// whose fan is already closed) or a complex edge (a halfedge that already has
+// a face) are rejected.  Remaining boundary halfedges are linked into loops
+// and boundary vertices point at their boundary outgoing halfedge.
+void build_topology(tm_mesh& M, const int* faces, int m)
+{
+    const int n = M.nv;
+    M.v_he.assign(n, -1);
+    M.he_to.clear(); M.he_next.clear(); M.he_prev.clear(); M.he_face.clear(); M.f_he.clear();
+    if (n < 3 || m == 0) return;
+    std::unordered_map<uint64_t, int> he_of;
+    he_of.reserve((size_t)m * 4);
+    auto key = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
+    std::vector<int> n_out(n, 0), n_bout(n, 0);
+    for (int f = 0; f < m; f++) {
+        int vh[3], nvh = 0;
+        for (int j = 0; j < 3; j++) {
+            int c = faces[3 * f + j];
+            if (c >= 0 && c < n) vh[nvh++] = c;
+        }
+        if (nvh < 3) continue;
+        int hh[3];
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; i++) {
+            int a = vh[i], b = vh[(i + 1) % 3];
+            if (n_out[a] > 0 && n_bout[a] == 0) ok = false;
+            auto it = he_of.find(key(a, b));
+            hh[i] = it == he_of.end() ? -1 : it->second;
+            if (hh[i] >= 0 && M.he_face[hh[i]] >= 0) ok = false;
+        }
+        if (!ok) continue;
+        int fid = (int)M.f_he.size();
+        for (int i = 0; i < 3; i++) {
+            if (hh[i] >= 0) continue;
+            int a = vh[i], b = vh[(i + 1) % 3];
+            int e = (int)M.he_to.size() / 2;
+            M.he_to.push_back(b); M.he_to.push_back(a);
+            M.he_next.push_back(-1); M.he_next.push_back(-1);
+            M.he_prev.push_back(-1); M.he_prev.push_back(-1);
+            M.he_face.push_back(-1); M.he_face.push_back(-1);
+            he_of[key(a, b)] = 2 * e;
+            he_of[key(b, a)] = 2 * e + 1;
+            hh[i] = 2 * e;
+            n_out[a]++; n_bout[a]++;
+            n_out[b]++; n_bout[b]++;
+            if (M.v_he[a] < 0) M.v_he[a] = 2 * e;
+            if (M.v_he[b] < 0) M.v_he[b] = 2 * e + 1;
+        }
+        for (int i = 0; i < 3; i++) {
+            M.he_face[hh[i]] = fid;
+            M.he_next[hh[i]] = hh[(i + 1) % 3];
+            M.he_prev[hh[(i + 1) % 3]] = hh[i];
+            n_bout[vh[i]]--;
+        }
+        M.f_he.push_back(hh[2]);
+    }
+    const int nh = (int)M.he_to.size();
+    std::vector<int> bout(n, -1);
+    for (int h = 0; h < nh; h++)
+        if (M.he_face[h] < 0) bout[M.he_to[h ^ 1]] = h;
+    for (int h = 0; h < nh; h++) {
+        if (M.he_face[h] < 0) {
+            int nx = bout[M.he_to[h]];
+            M.he_next[h] = nx;
+            if (nx >= 0) M.he_prev[nx] = h;
+        }
+    }
+    M.has_boundary = false;
+    for (int v = 0; v < n; v++) {
+        if (bout[v] >= 0) {
+            M.v_he[v] = bout[v];
+            M.has_boundary = true;
+        }
+    }
+    M.nh = nh;
+    M.nf = (int)M.f_he.size();
+}
+
+void ensure_device(tm_mesh& M)
+{
+    ensure_runtime();
+    if (M.dev) return;
+    M.x.alloc((size_t)M.nv * 3);
+    M.d_to.alloc(M.nh);
+    M.d_next.alloc(M.nh);
+    M.d_prev.alloc(M.nh);
+    M.d_face.alloc(M.nh);
+    M.d_vhe.alloc(M.nv);
+    M.d_fhe.alloc(M.nf);
+    M.ring.alloc((size_t)kMaxValence * M.nv);
+    M.meta.alloc(M.nv);
+    M.flag.alloc(4);
+    h2d(M.d_to.p, M.he_to.data(), sizeof(int) * M.nh);
+    h2d(M.d_next.p, M.he_next.data(), sizeof(int) * M.nh);
+    h2d(M.d_prev.p, M.he_prev.data(), sizeof(int) * M.nh);
+    h2d(M.d_face.p, M.he_face.data(), sizeof(int) * M.nh);
+    h2d(M.d_vhe.p, M.v_he.data(), sizeof(int) * M.nv);
+    h2d(M.d_fhe.p, M.f_he.data(), sizeof(int) * M.nf);
+    HIPCHECK(hipMemsetAsync(M.flag.p, 0, sizeof(int) * 4, g_rt.stream));
+    launch_build_ring(M.view(), M.ring.p, M.meta.p, M.flag.p, g_rt.stream);
+    check_launch();
+    int ovf = 0;
+    d2h(&ovf, M.flag.p, sizeof(int));
+    sync();
+    if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
+    M.dev = true;
+}
+
+void upload_points(tm_mesh& M)
+{
+    ensure_device(M);
+    h2d(M.x.p, M.points, sizeof(double) * 3 * M.nv);
+}
+
+void refresh_host_topology(tm_mesh& M)
+{
+    if (!M.host_stale) return;
+    d2h(M.he_to.data(), M.d_to.p, sizeof(int) * M.nh);
+    d2h(M.he_next.data(), M.d_next.p, sizeof(int) * M.nh);
+    d2h(M.he_prev.data(), M.d_prev.p, sizeof(int) * M.nh);
+    d2h(M.he_face.data(), M.d_face.p, sizeof(int) * M.nh);
+    d2h(M.v_he.data(), M.d_vhe.p, sizeof(int) * M.nv);
+    d2h(M.f_he.data(), M.d_fhe.p, sizeof(int) * M.nf);
+    sync();
+    M.host_stale = false;
+}
+
+}    // namespace
+
+// ---------------------------------------------------------- nlist ---------
+struct tm_nlist {
+    int kind = 0;    // 0 cell, 1 verlet
+    int exclusion = 0;
+    double rlist = 0.1;
+    NListView v{};
+    DevBuf<int> cell_start, cell_verts, vert_cell, counts, fill, scan_tmp, adj_start, adj, adj_counts;
+    // distance-matrix scratch
+    DevBuf<int> dm_counts, dm_off;
+    DevBuf<double> dm_d;
+    DevBuf<int> dm_i, dm_j;
+    DevBuf<double> box, pd_dist;
+    DevBuf<int> pd_j, pd_count;
+    int nv = 0;
+    NListView view() const { return v; }
+};
+
+namespace {
+
+void scan(const int* in, int* out, int n, DevBuf<int>& tmp)
+{
+    tmp.alloc(scan_block_sums_size(n) + 1);
+    launch_exclusive_scan(in, out, n, tmp.p, g_rt.stream);
+    check_launch();
+}
+
+// CellList ctor (cell_list.h:43-87) and, for kind 1, VerletList ctor
+// (verlet_list.h:18-35), on the mesh's current device positions.
+void build_nlist(tm_nlist& L, tm_mesh& M)
+{
+    if (!(L.rlist > 0) || !std::isfinite(L.rlist)) throw TmError(TM_E_INVALID, "rlist must be positive");
+    const int n = M.nv;
+    L.nv = n;
+    L.box.alloc(6);
+    launch_box(M.x.p, n, L.box.p, g_rt.stream);
+    check_launch();
+    double box[6];
+    d2h(box, L.box.p, sizeof(box));
+    sync();
+    const double box_eps = 1.0e-6;
+    NListView v{};
+    v.kind = 0;
+    v.exclusion = L.exclusion;
+    long long ncells = 1;
+    for (int k = 0; k < 3; k++) {
+        double dim = box[3 + k] - box[k] + box_eps;
+        int nk = int(dim / L.rlist);
+        v.box_min[k] = box[k];
+        v.r_list[k] = (nk > 0) ? dim / nk : dim;
+        v.shape[k] = (nk > 0) ? nk : 1;
+        ncells *= v.shape[k];
+    }
+    if (ncells > (1ll << 28))
+        throw TmError(TM_E_UNSUPPORTED, "cell grid too large (" + std::to_string(ncells) + " cells)");
+    v.strides[0] = 1;
+    v.strides[1] = v.shape[0];
+    v.strides[2] = v.shape[0] * v.shape[1];
+    const int nc = (int)ncells;
+    L.counts.alloc(nc + 1);
+    L.cell_start.alloc(nc + 1);
+    L.fill.alloc(nc);
+    L.vert_cell.alloc(n);
+    L.cell_verts.alloc(n);
+    HIPCHECK(hipMemsetAsync(L.counts.p, 0, sizeof(int) * (nc + 1), g_rt.stream));
+    HIPCHECK(hipMemsetAsync(L.fill.p, 0, sizeof(int) * nc, g_rt.stream));
+    launch_cell_ids(M.x.p, n, v, L.vert_cell.p, L.counts.p, g_rt.stream);
+    check_launch();
+    scan(L.counts.p, L.cell_start.p, nc + 1, L.scan_tmp);
+    launch_cell_scatter(L.vert_cell.p, n, L.cell_start.p, L.fill.p, L.cell_verts.p, g_rt.stream);
+    check_launch();
+    launch_cell_sort_segments(L.vert_cell.p, n, L.cell_start.p, L.cell_verts.p, g_rt.stream);
+    check_launch();
+    v.cell_start = L.cell_start.p;
+    v.cell_verts = L.cell_verts.p;
+    v.vert_cell = L.vert_cell.p;
+    if (L.kind == 1) {
+        L.adj_counts.alloc(n + 1);
+        L.adj_start.alloc(n + 1);
+        HIPCHECK(hipMemsetAsync(L.adj_counts.p, 0, sizeof(int) * (n + 1), g_rt.stream));
+        launch_verlet_count(M.view(), v, L.rlist, L.adj_counts.p, g_rt.stream);
+        check_launch();
+        scan(L.adj_counts.p, L.adj_start.p, n + 1, L.scan_tmp);
+        int total = 0;
+        d2h(&total, L.adj_start.p + n, sizeof(int));
+        sync();
+        L.adj.alloc(total);
+        launch_verlet_fill(M.view(), v, L.rlist, L.adj_start.p, L.adj.p, g_rt.stream);
+        check_launch();
+        v.kind = 1;
+        v.adj_start = L.adj_start.p;
+        v.adj = L.adj.p;
+    }
+    L.v = v;
+    sync();
+}
+
+void check_params(const tm_params& p)
+{
+    if (p.bond_type != 0 && p.bond_type != 1) throw TmError(TM_E_INVALID, "Unknown bond potential");
+    if (p.rep_nlist != 0 && p.rep_nlist != 1) throw TmError(TM_E_INVALID, "Unknown neighbour search algorithm.");
+    if (p.rep_exclusion < 0 || p.rep_exclusion > 2) throw TmError(TM_E_INVALID, "Unsupported exclusion level");
+}
+
+tm_nlist* make_nlist(tm_mesh& M, const tm_params& p)
+{
+    check_params(p);
+    auto L = std::make_unique<tm_nlist>();
+    L->kind = p.rep_nlist;
+    L->exclusion = p.rep_exclusion;
+    L->rlist = p.rep_rlist;
+    build_nlist(*L, M);
+    return L.release();
+}
+
+}    // namespace
+
+// --------------------------------------------------------- energy ---------
+struct tm_energy {
+    tm_params params{};
+    tm_props initial{};
+    int precision = TM_PREC_FP64;
+    tm_nlist* nlist = nullptr;
+    int nv = 0;
+    DevBuf<double> q, partials, grad, vprops;
+    DevBuf<unsigned> counter;
+    DevBuf<Globals> globals;
+    DevBuf<int> flip_result, proposals;
+    DevBuf<double> uniforms;
+    ~tm_energy() { delete nlist; }
+};
+
+namespace {
+
+EParams eparams_of(const tm_energy& E)
+{
+    const tm_params& P = E.params;
+    EParams p{};
+    p.kappa_b = P.kappa_b; p.kappa_a = P.kappa_a; p.kappa_v = P.kappa_v;
+    p.kappa_c = P.kappa_c; p.kappa_t = P.kappa_t; p.kappa_r = P.kappa_r;
+    p.bond_lc0 = P.bond_lc0; p.bond_lc1 = P.bond_lc1; p.bond_a0 = P.bond_a0;
+    p.bond_r = P.bond_r; p.bond_type = P.bond_type;
+    p.rep_lc1 = P.rep_lc1; p.rep_r = P.rep_r; p.rep_exclusion = P.rep_exclusion;
+    // interpolate_reference_properties (energy.cpp:30-45)
+    double lam = P.cont_lambda;
+    p.ref_area = (1.0 - lam * (1.0 - P.area_frac)) * E.initial.area;
+    p.ref_volume = (1.0 - lam * (1.0 - P.volume_frac)) * E.initial.volume;
+    p.ref_curvature = (1.0 - lam * (1.0 - P.curvature_frac)) * E.initial.curvature;
+    return p;
+}
+
+void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
+{
+    if (E.nv == M.nv && E.q.p) return;
+    E.nv = M.nv;
+    E.q.alloc(M.nv);
+    E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
+    E.grad.alloc((size_t)M.nv * 3);
+    E.counter.alloc(1);
+    E.globals.alloc(1);
+    E.flip_result.alloc(4);
+    HIPCHECK(hipMemsetAsync(E.counter.p, 0, sizeof(unsigned), g_rt.stream));
+}
+
+void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
+{
+    ensure_energy_buffers(E, M);
+    if (E.nlist->nv != M.nv) throw TmError(TM_E_INVALID, "mesh does not match the energy manager");
+    if (M.nv == 0) {
+        HIPCHECK(hipMemsetAsync(E.globals.p, 0, sizeof(Globals), g_rt.stream));
+        return;
+    }
+    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, E.q.p, vprops, E.partials.p, E.counter.p,
+                 E.globals.p, g_rt.stream);
+    check_launch();
+}
+
+void run_pass2(tm_energy& E, tm_mesh& M, int mixed)
+{
+    if (M.nv == 0) return;
+    launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, E.q.p, E.globals.p, E.grad.p, g_rt.stream);
+    check_launch();
+}
+
+Globals read_globals(tm_energy& E)
+{
+    Globals g;
+    d2h(&g, E.globals.p, sizeof(Globals));
+    sync();
+    return g;
+}
+
+tm_props props_of(const Globals& g)
+{
+    return tm_props{g.props[0], g.props[1], g.props[2], g.props[3], g.props[4], g.props[5]};
+}
+
+// trimem_energy (kernel.h:110-122) on host-supplied properties
+double energy_from(const tm_energy& E, const tm_props& pr)
+{
+    EParams p = eparams_of(E);
+    double e = 0.0, d;
+    d = pr.area / p.ref_area - 1.0;
+    e += p.kappa_a * d * d;
+    d = pr.volume / p.ref_volume - 1.0;
+    e += p.kappa_v * d * d;
+    d = pr.curvature / p.ref_curvature - 1.0;
+    e += p.kappa_c * d * d;
+    e += p.kappa_t * pr.tethering;
+    e += p.kappa_r * pr.repulsion;
+    e += p.kappa_b * pr.bending;
+    return e;
+}
+
+void require(bool c, const char* msg)
+{
+    if (!c) throw TmError(TM_E_INVALID, msg);
+}
+
+}    // namespace
+
+// ================================================================ C ABI ====
+extern "C" {
+
+const char* tm_last_error(void) { return g_err.c_str(); }
+
+int tm_device_count(int* count)
+{
+    API_BEGIN
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    API_END
+}
+
+int tm_set_device(int device)
+{
+    API_BEGIN
+    require(!g_rt.init || g_rt.device == device, "device already selected");
+    g_rt.device = device;
+    API_END
+}
+
+int tm_mesh_create(const double* points, int n_points, const int* faces, int n_faces, tm_mesh** out)
+{
+    API_BEGIN
+    require(out != nullptr, "null output");
+    require(n_points >= 0 && n_faces >= 0, "negative size");
+    auto M = std::make_unique<tm_mesh>();
+    M->nv = n_points;
+    size_t bytes = sizeof(double) * 3 * std::max(n_points, 1);
+    // pinned host memory when a device is present (fast uploads); plain
+    // memory otherwise so that meshes can be built on a host without a GPU
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0 &&
+        hipHostMalloc((void**)&M->points, bytes, hipHostMallocDefault) == hipSuccess) {
+        M->pinned = true;
+    } else {
+        (void)hipGetLastError();
+        M->points = (double*)std::malloc(bytes);
+        if (!M->points) throw TmError(TM_E_INVALID, "out of host memory");
+    }
+    if (n_points) std::memcpy(M->points, points, sizeof(double) * 3 * n_points);
+    build_topology(*M, faces, n_faces);
+    *out = M.release();
+    API_END
+}
+
+int tm_mesh_destroy(tm_mesh* mesh)
+{
+    delete mesh;
+    return TM_OK;
+}
+
+int tm_mesh_counts(const tm_mesh* mesh, int* nv, int* ne, int* nf)
+{
+    API_BEGIN
+    if (nv) *nv = mesh->nv;
+    if (ne) *ne = mesh->nh / 2;
+    if (nf) *nf = mesh->nf;
+    API_END
+}
+
+double* tm_mesh_points(tm_mesh* mesh) { return mesh->points; }
+
+int tm_mesh_fv_indices(tm_mesh* mesh, int* out)
+{
+    API_BEGIN
+    refresh_host_topology(*mesh);
+    for (int f = 0; f < mesh->nf; f++) {
+        int h = mesh->f_he[f];
+        out[3 * f] = mesh->he_to[h];
+        h = mesh->he_next[h];
+        out[3 * f + 1] = mesh->he_to[h];
+        h = mesh->he_next[h];
+        out[3 * f + 2] = mesh->he_to[h];
+    }
+    API_END
+}
+
+int tm_mesh_next_halfedge(tm_mesh* mesh, int he, int* out)
+{
+    API_BEGIN
+    refresh_host_topology(*mesh);
+    *out = (he >= 0 && he < mesh->nh) ? mesh->he_next[he] : -1;
+    API_END
+}
+
+int tm_mesh_topology(tm_mesh* mesh, int* he_to, int* he_next, int* he_face, int* v_he, int* f_he)
+{
+    API_BEGIN
+    refresh_host_topology(*mesh);
+    if (he_to) std::memcpy(he_to, mesh->he_to.data(), sizeof(int) * mesh->nh);
+    if (he_next) std::memcpy(he_next, mesh->he_next.data(), sizeof(int) * mesh->nh);
+    if (he_face) std::memcpy(he_face, mesh->he_face.data(), sizeof(int) * mesh->nh);
+    if (v_he) std::memcpy(v_he, mesh->v_he.data(), sizeof(int) * mesh->nv);
+    if (f_he) std::memcpy(f_he, mesh->f_he.data(), sizeof(int) * mesh->nf);
+    API_END
+}
+
+double* tm_mesh_device_points(tm_mesh* mesh)
+{
+    try {
+        ensure_device(*mesh);
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return nullptr;
+    }
+    return mesh->x.p;
+}
+
+int tm_mesh_upload_points(tm_mesh* mesh)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    sync();
+    API_END
+}
+
+int tm_mesh_download_points(tm_mesh* mesh)
+{
+    API_BEGIN
+    ensure_device(*mesh);
+    d2h(mesh->points, mesh->x.p, sizeof(double) * 3 * mesh->nv);
+    sync();
+    API_END
+}
+
+int tm_halfedge_geometry(tm_mesh* mesh, int he, double* out47)
+{
+    API_BEGIN
+    require(he >= 0 && he < mesh->nh, "halfedge out of range");
+    upload_points(*mesh);
+    DevBuf<double> d;
+    d.alloc(47);
+    launch_halfedge_geometry(mesh->view(), he, d.p, g_rt.stream);
+    check_launch();
+    d2h(out47, d.p, sizeof(double) * 47);
+    sync();
+    API_END
+}
+
+static int sum_geometry(tm_mesh* mesh, int what, double* out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    int n = what == 0 ? mesh->nf : mesh->nh / 2;
+    DevBuf<double> part, res;
+    DevBuf<unsigned> cnt;
+    part.alloc((n + 255) / 256 + 1);
+    res.alloc(1);
+    cnt.alloc(1);
+    HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), g_rt.stream));
+    HIPCHECK(hipMemsetAsync(res.p, 0, sizeof(double), g_rt.stream));
+    if (n > 0) {
+        launch_sum_geometry(mesh->view(), what, part.p, cnt.p, res.p, g_rt.stream);
+        check_launch();
+    }
+    d2h(out, res.p, sizeof(double));
+    sync();
+    API_END
+}
+
+int tm_area(tm_mesh* mesh, double* out) { return sum_geometry(mesh, 0, out); }
+int tm_edges_length(tm_mesh* mesh, double* out) { return sum_geometry(mesh, 1, out); }
+
+// ---- energy ----
+int tm_energy_create(tm_mesh* mesh, const tm_params* params, tm_energy** out)
+{
+    API_BEGIN
+    check_params(*params);
+    auto E = std::make_unique<tm_energy>();
+    E->params = *params;
+    upload_points(*mesh);
+    E->nlist = make_nlist(*mesh, E->params);
+    // initial_props = properties(mesh) (energy.cpp:27); reference values
+    // are not needed for the sums, unit placeholders keep the factors finite
+    E->initial = tm_props{1, 1, 1, 0, 0, 0};
+    run_pass1(*E, *mesh, 0, nullptr);
+    E->initial = props_of(read_globals(*E));
+    *out = E.release();
+    API_END
+}
+
+int tm_energy_destroy(tm_energy* em)
+{
+    delete em;
+    return TM_OK;
+}
+
+int tm_energy_get_params(const tm_energy* em, tm_params* out)
+{
+    *out = em->params;
+    return TM_OK;
+}
+
+int tm_energy_initial_props(const tm_energy* em, tm_props* out)
+{
+    *out = em->initial;
+    return TM_OK;
+}
+
+int tm_energy_set_precision(tm_energy* em, int precision)
+{
+    API_BEGIN
+    require(precision == TM_PREC_FP64 || precision == TM_PREC_MIXED, "unknown precision");
+    em->precision = precision;
+    API_END
+}
+
+int tm_energy_properties(tm_energy* em, tm_mesh* mesh, tm_props* out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    run_pass1(*em, *mesh, em->precision, nullptr);
+    *out = props_of(read_globals(*em));
+    API_END
+}
+
+int tm_energy_vertex_properties(tm_energy* em, tm_mesh* mesh, double* out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    em->vprops.alloc((size_t)mesh->nv * 6);
+    run_pass1(*em, *mesh, em->precision, em->vprops.p);
+    d2h(out, em->vprops.p, sizeof(double) * 6 * mesh->nv);
+    sync();
+    API_END
+}
+
+int tm_energy_energy(tm_energy* em, tm_mesh* mesh, double* out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    run_pass1(*em, *mesh, em->precision, nullptr);
+    *out = read_globals(*em).energy;
+    API_END
+}
+
+int tm_energy_energy_props(tm_energy* em, const tm_props* props, double* out)
+{
+    *out = energy_from(*em, *props);
+    return TM_OK;
+}
+
+int tm_energy_gradient(tm_energy* em, tm_mesh* mesh, double* out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    run_pass1(*em, *mesh, em->precision, nullptr);
+    run_pass2(*em, *mesh, em->precision);
+    d2h(out, em->grad.p, sizeof(double) * 3 * mesh->nv);
+    sync();
+    API_END
+}
+
+int tm_energy_energy_gradient(tm_energy* em, tm_mesh* mesh, double* energy, double* grad)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    run_pass1(*em, *mesh, em->precision, nullptr);
+    run_pass2(*em, *mesh, em->precision);
+    Globals g;
+    d2h(&g, em->globals.p, sizeof(Globals));
+    d2h(grad, em->grad.p, sizeof(double) * 3 * mesh->nv);
+    sync();
+    *energy = g.energy;
+    API_END
+}
+
+int tm_energy_update_reference_properties(tm_energy* em)
+{
+    // energy.cpp:47-55
+    if (em->params.cont_lambda < 1.0) em->params.cont_lambda += em->params.cont_delta;
+    return TM_OK;
+}
+
+int tm_energy_update_repulsion(tm_energy* em, tm_mesh* mesh)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    tm_nlist* fresh = make_nlist(*mesh, em->params);
+    delete em->nlist;
+    em->nlist = fresh;
+    API_END
+}
+
+int tm_energy_eval_device(tm_energy* em, tm_mesh* mesh, int flags)
+{
+    API_BEGIN
+    ensure_device(*mesh);
+    if (flags & TM_EVAL_PROPS) run_pass1(*em, *mesh, em->precision, nullptr);
+    if (flags & TM_EVAL_GRAD) run_pass2(*em, *mesh, em->precision);
+    API_END
+}
+
+const double* tm_energy_device_gradient(tm_energy* em) { return em->grad.p; }
+const double* tm_energy_device_globals(tm_energy* em) { return (const double*)em->globals.p; }
+
+int tm_stream(void** out)
+{
+    API_BEGIN
+    ensure_runtime();
+    *out = (void*)g_rt.stream;
+    API_END
+}
+
+int tm_sync(void)
+{
+    API_BEGIN
+    ensure_runtime();
+    sync();
+    API_END
+}
+
+// ---- neighbour lists ----
+int tm_nlist_create(tm_mesh* mesh, const tm_params* params, tm_nlist** out)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    *out = make_nlist(*mesh, *params);
+    API_END
+}
+
+int tm_nlist_destroy(tm_nlist* nl)
+{
+    delete nl;
+    return TM_OK;
+}
+
+int tm_energy_nlist(tm_energy* em, tm_nlist** out)
+{
+    *out = em->nlist;
+    return TM_OK;
+}
+
+int tm_nlist_distance_matrix(tm_nlist* nl, tm_mesh* mesh, double dmax, double* d, int* i, int* j, long cap,
+                             long* count)
+{
+    API_BEGIN
+    require(nl->nv == mesh->nv, "mesh does not match the neighbour list");
+    upload_points(*mesh);
+    const int n = mesh->nv;
+    nl->dm_counts.alloc(n + 1);
+    nl->dm_off.alloc(n + 1);
+    HIPCHECK(hipMemsetAsync(nl->dm_counts.p, 0, sizeof(int) * (n + 1), g_rt.stream));
+    launch_dm_count(mesh->view(), nl->view(), dmax, nl->dm_counts.p, g_rt.stream);
+    check_launch();
+    scan(nl->dm_counts.p, nl->dm_off.p, n + 1, nl->scan_tmp);
+    int total = 0;
+    d2h(&total, nl->dm_off.p + n, sizeof(int));
+    sync();
+    *count = total;
+    if (d && i && j && total > 0) {
+        nl->dm_d.alloc(total);
+        nl->dm_i.alloc(total);
+        nl->dm_j.alloc(total);
+        launch_dm_fill(mesh->view(), nl->view(), dmax, nl->dm_off.p, nl->dm_d.p, nl->dm_i.p, nl->dm_j.p,
+                       g_rt.stream);
+        check_launch();
+        long k = std::min<long>(cap, total);
+        d2h(d, nl->dm_d.p, sizeof(double) * k);
+        d2h(i, nl->dm_i.p, sizeof(int) * k);
+        d2h(j, nl->dm_j.p, sizeof(int) * k);
+        sync();
+    }
+    API_END
+}
+
+int tm_nlist_point_distances(tm_nlist* nl, tm_mesh* mesh, int pid, double dmax, double* dist3, int* jdx, long cap,
+                             long* count)
+{
+    API_BEGIN
+    require(pid >= 0 && pid < mesh->nv, "pid out of range");
+    require(nl->nv == mesh->nv, "mesh does not match the neighbour list");
+    upload_points(*mesh);
+    // first pass counts, second pass fills a buffer of that size
+    nl->pd_count.alloc(1);
+    launch_point_distances(mesh->view(), nl->view(), pid, dmax, 0, nl->pd_count.p, nullptr, nullptr, g_rt.stream);
+    check_launch();
+    int c = 0;
+    d2h(&c, nl->pd_count.p, sizeof(int));
+    sync();
+    *count = c;
+    if (dist3 && jdx && c > 0) {
+        nl->pd_dist.alloc((size_t)c * 3);
+        nl->pd_j.alloc(c);
+        launch_point_distances(mesh->view(), nl->view(), pid, dmax, c, nl->pd_count.p, nl->pd_dist.p, nl->pd_j.p,
+                               g_rt.stream);
+        check_launch();
+        long k = std::min<long>(cap, c);
+        d2h(dist3, nl->pd_dist.p, sizeof(double) * 3 * k);
+        d2h(jdx, nl->pd_j.p, sizeof(int) * k);
+        sync();
+    }
+    API_END
+}
+
+// ---- flips ----
+static int flip_impl(tm_energy* em, tm_mesh* mesh, int nflips, const int* proposals, const double* uniforms,
+                     int n_uniforms, uint64_t seed, int injected, int* accepted, int* n_used)
+{
+    API_BEGIN
+    upload_points(*mesh);
+    refresh_host_topology(*mesh);
+    // initial properties and energy (flips.cpp:30-31), FP64
+    run_pass1(*em, *mesh, 0, nullptr);
+    if (injected && nflips > 0) {
+        em->proposals.alloc(nflips);
+        h2d(em->proposals.p, proposals, sizeof(int) * nflips);
+        em->uniforms.alloc(std::max(n_uniforms, 1));
+        if (n_uniforms > 0) h2d(em->uniforms.p, uniforms, sizeof(double) * n_uniforms);
+    }
+    launch_flip_serial(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips,
+                       injected ? em->proposals.p : nullptr, injected ? em->uniforms.p : nullptr, n_uniforms, seed,
+                       injected, em->globals.p, em->flip_result.p, g_rt.stream);
+    check_launch();
+    HIPCHECK(hipMemsetAsync(mesh->flag.p, 0, sizeof(int), g_rt.stream));
+    launch_build_ring(mesh->view(), mesh->ring.p, mesh->meta.p, mesh->flag.p, g_rt.stream);
+    check_launch();
+    int res[4] = {0, 0, 0, 0};
+    d2h(res, em->flip_result.p, sizeof(int) * 3);
+    int ovf = 0;
+    d2h(&ovf, mesh->flag.p, sizeof(int));
+    sync();
+    mesh->host_stale = true;
+    if (res[2] == 1) throw TmError(TM_E_INVALID, "flip proposal out of range");
+    if (res[2] == 2) throw TmError(TM_E_INVALID, "uniform stream exhausted");
+    if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
+    if (accepted) *accepted = res[0];
+    if (n_used) *n_used = res[1];
+    API_END
+}
+
+int tm_flip_serial(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int* accepted)
+{
+    // flip_ratio > 1 is not rejected here, like flips.cpp:24-25
+    int nflips = (int)((mesh->nh / 2) * flip_ratio);
+    return flip_impl(em, mesh, nflips, nullptr, nullptr, 0, seed, 0, accepted, nullptr);
+}
+
+int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const int* proposals,
+                            const double* uniforms, int n_uniforms, int* accepted, int* n_used)
+{
+    return flip_impl(em, mesh, n_proposals, proposals, uniforms, n_uniforms, 0, 1, accepted, n_used);
+}
+
+}    // extern "C"

@@ -1,0 +1,58 @@
+// tm_internal.h -- launchers shared between the library's translation units.
+#pragma once
+
+#include "tm_common.h"
+
+namespace tmh {
+
+// energy passes (tm_energy.hip)
+int pass1_blocks(int nv);
+void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, double* q, double* vprops,
+                  double* partials, unsigned* counter, Globals* g, hipStream_t s);
+void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const double* q,
+                  const Globals* g, double* grad, hipStream_t s);
+void launch_halfedge_geometry(const MeshView& m, int h, double* out, hipStream_t s);
+void launch_sum_geometry(const MeshView& m, int what, double* partials, unsigned* counter, double* result,
+                         hipStream_t s);
+
+// topology helpers (tm_mesh_kernels.hip)
+// Rebuild ring table and meta from the halfedge arrays; sets *overflow to 1
+// if a vertex has more than kMaxValence neighbours.
+void launch_build_ring(const MeshView& m, int* ring, int* meta, int* overflow, hipStream_t s);
+
+// neighbour lists (tm_nlist.hip)
+void launch_box(const double* x, int n, double* box6, hipStream_t s);
+void launch_cell_ids(const double* x, int n, const NListView& nl, int* vert_cell, int* counts, hipStream_t s);
+void launch_exclusive_scan(const int* in, int* out, int n, int* block_sums, hipStream_t s);
+int scan_block_sums_size(int n);
+void launch_cell_scatter(const int* vert_cell, int n, const int* cell_start, int* fill, int* cell_verts,
+                         hipStream_t s);
+void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, int* cell_verts, hipStream_t s);
+// distance_matrix (cell list or verlet) in two phases: counts then fill
+void launch_dm_count(const MeshView& m, const NListView& nl, double dmax, int* counts, hipStream_t s);
+void launch_dm_fill(const MeshView& m, const NListView& nl, double dmax, const int* offsets, double* d, int* i,
+                    int* j, hipStream_t s);
+// point_distances of one vertex in the reference's stencil order
+void launch_point_distances(const MeshView& m, const NListView& nl, int pid, double dmax, int cap, int* count,
+                            double* dist3, int* jdx, hipStream_t s);
+// verlet adjacency from a cell list: counts, then fill (ascending ids)
+void launch_verlet_count(const MeshView& m, const NListView& cl, double rlist, int* counts, hipStream_t s);
+void launch_verlet_fill(const MeshView& m, const NListView& cl, double rlist, const int* offsets, int* adj,
+                        hipStream_t s);
+
+// flips (tm_flips.hip)
+struct FlipMeshView {
+    double* x;
+    int* he_to;
+    int* he_next;
+    int* he_prev;
+    int* he_face;
+    int* v_he;
+    int* f_he;
+    int nv, nh, nf;
+};
+void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips,
+                        const int* proposals, const double* uniforms, int n_uniforms, uint64_t seed, int injected,
+                        const Globals* g0, int* result /* [accepted, used, error] */, hipStream_t s);
+
+}    // namespace tmh
