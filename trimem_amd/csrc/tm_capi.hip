@@ -117,6 +117,17 @@ struct tm_mesh {
     bool dev = false;
     DevBuf<double> x;
     DevBuf<int> d_to, d_next, d_prev, d_face, d_vhe, d_fhe, ring, meta, flag;
+    // vertices evaluated by the half-edge walker: boundary, isolated or with
+    // fewer than three neighbours (flips preserve this set on valid meshes)
+    DevBuf<int> generic;
+    int n_generic = 0;
+    // Internal vertex order (Morton order of the creation-time positions):
+    // perm[i] = user id of internal vertex i, iperm[u] = internal id of user
+    // vertex u.  Topology and device arrays use internal ids; every id or
+    // per-vertex row that crosses the API is mapped back to user ids.
+    std::vector<int> perm, iperm;
+    DevBuf<int> d_perm, d_iperm;
+    DevBuf<double> stage;    // user-order rows for uploads and downloads
 
     MeshView view() const
     {
@@ -219,6 +230,52 @@ void build_topology(tm_mesh& M, const int* faces, int m)
     M.nf = (int)M.f_he.size();
 }
 
+// Morton (Z-order) key of each vertex on a 1024^3 grid over the bounding box,
+// ties broken by id; vertices close in space get close internal ids, which
+// keeps the gathers of one wavefront in few cache lines.
+void reorder_vertices(tm_mesh& M)
+{
+    const int n = M.nv;
+    M.perm.resize(n);
+    M.iperm.resize(n);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v = 0; v < n; v++)
+        for (int k = 0; k < 3; k++) {
+            double c = M.points[3 * v + k];
+            if (c < lo[k]) lo[k] = c;
+            if (c > hi[k]) hi[k] = c;
+        }
+    auto spread = [](uint64_t x) {
+        x &= 0x3ff;
+        x = (x | (x << 16)) & 0x030000ffull;
+        x = (x | (x << 8)) & 0x0300f00full;
+        x = (x | (x << 4)) & 0x030c30c3ull;
+        x = (x | (x << 2)) & 0x09249249ull;
+        return x;
+    };
+    std::vector<uint64_t> key(n);
+    for (int v = 0; v < n; v++) {
+        uint64_t code = 0;
+        for (int k = 0; k < 3; k++) {
+            double span = hi[k] - lo[k];
+            double t = span > 0 ? (M.points[3 * v + k] - lo[k]) / span : 0.0;
+            if (!(t >= 0)) t = 0;    // also catches NaN
+            if (t > 1) t = 1;
+            code |= spread((uint64_t)(t * 1023.0)) << k;
+        }
+        key[v] = (code << 32) | (uint32_t)v;
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < n; i++) {
+        M.perm[i] = (int)(key[i] & 0xffffffffu);
+        M.iperm[M.perm[i]] = i;
+    }
+    for (int h = 0; h < M.nh; h++) M.he_to[h] = M.iperm[M.he_to[h]];
+    std::vector<int> vhe(n);
+    for (int i = 0; i < n; i++) vhe[i] = M.v_he[M.perm[i]];
+    M.v_he.swap(vhe);
+}
+
 void ensure_device(tm_mesh& M)
 {
     ensure_runtime();
@@ -232,6 +289,11 @@ void ensure_device(tm_mesh& M)
     M.d_fhe.alloc(M.nf);
     M.ring.alloc((size_t)kMaxValence * M.nv);
     M.meta.alloc(M.nv);
+    M.d_perm.alloc(M.nv);
+    M.d_iperm.alloc(M.nv);
+    M.stage.alloc((size_t)M.nv * 6);
+    h2d(M.d_perm.p, M.perm.data(), sizeof(int) * M.nv);
+    h2d(M.d_iperm.p, M.iperm.data(), sizeof(int) * M.nv);
     M.flag.alloc(4);
     h2d(M.d_to.p, M.he_to.data(), sizeof(int) * M.nh);
     h2d(M.d_next.p, M.he_next.data(), sizeof(int) * M.nh);
@@ -239,6 +301,23 @@ void ensure_device(tm_mesh& M)
     h2d(M.d_face.p, M.he_face.data(), sizeof(int) * M.nh);
     h2d(M.d_vhe.p, M.v_he.data(), sizeof(int) * M.nv);
     h2d(M.d_fhe.p, M.f_he.data(), sizeof(int) * M.nf);
+    std::vector<int> gl;
+    for (int v = 0; v < M.nv; v++) {
+        int h0 = M.v_he[v];
+        if (h0 < 0 || M.he_face[h0] < 0) {
+            gl.push_back(v);
+            continue;
+        }
+        int n = 0, h = h0;
+        do {
+            n++;
+            h = M.he_next[h ^ 1];
+        } while (h != h0 && n < 4);
+        if (n < 3) gl.push_back(v);
+    }
+    M.n_generic = (int)gl.size();
+    M.generic.alloc(std::max<size_t>(gl.size(), 1));
+    h2d(M.generic.p, gl.data(), sizeof(int) * gl.size());
     HIPCHECK(hipMemsetAsync(M.flag.p, 0, sizeof(int) * 4, g_rt.stream));
     launch_build_ring(M.view(), M.ring.p, M.meta.p, M.flag.p, g_rt.stream);
     check_launch();
@@ -252,7 +331,20 @@ void ensure_device(tm_mesh& M)
 void upload_points(tm_mesh& M)
 {
     ensure_device(M);
-    h2d(M.x.p, M.points, sizeof(double) * 3 * M.nv);
+    if (M.nv == 0) return;
+    h2d(M.stage.p, M.points, sizeof(double) * 3 * M.nv);
+    launch_gather_rows(M.stage.p, M.d_perm.p, M.x.p, M.nv, 3, g_rt.stream);
+    check_launch();
+}
+
+// rows of an internal-order device array -> user order in the stage buffer
+// -> host
+void download_rows(tm_mesh& M, const double* d_src, int width, double* host)
+{
+    if (M.nv == 0) return;
+    launch_gather_rows(d_src, M.d_iperm.p, M.stage.p, M.nv, width, g_rt.stream);
+    check_launch();
+    d2h(host, M.stage.p, sizeof(double) * width * M.nv);
 }
 
 void refresh_host_topology(tm_mesh& M)
@@ -281,7 +373,8 @@ struct tm_nlist {
     DevBuf<int> dm_counts, dm_off;
     DevBuf<double> dm_d;
     DevBuf<int> dm_i, dm_j;
-    DevBuf<double> box, pd_dist;
+    DevBuf<double> box, box_part, pd_dist;
+    DevBuf<unsigned> box_counter;
     DevBuf<int> pd_j, pd_count;
     int nv = 0;
     NListView view() const { return v; }
@@ -304,7 +397,12 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     const int n = M.nv;
     L.nv = n;
     L.box.alloc(6);
-    launch_box(M.x.p, n, L.box.p, g_rt.stream);
+    L.box_part.alloc(box_scratch_doubles());
+    if (!L.box_counter.p) {
+        L.box_counter.alloc(1);
+        HIPCHECK(hipMemsetAsync(L.box_counter.p, 0, sizeof(unsigned), g_rt.stream));
+    }
+    launch_box(M.x.p, n, L.box_part.p, L.box_counter.p, L.box.p, g_rt.stream);
     check_launch();
     double box[6];
     d2h(box, L.box.p, sizeof(box));
@@ -340,11 +438,12 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     scan(L.counts.p, L.cell_start.p, nc + 1, L.scan_tmp);
     launch_cell_scatter(L.vert_cell.p, n, L.cell_start.p, L.fill.p, L.cell_verts.p, g_rt.stream);
     check_launch();
-    launch_cell_sort_segments(L.vert_cell.p, n, L.cell_start.p, L.cell_verts.p, g_rt.stream);
+    launch_cell_sort_segments(L.vert_cell.p, n, L.cell_start.p, M.d_perm.p, L.cell_verts.p, g_rt.stream);
     check_launch();
     v.cell_start = L.cell_start.p;
     v.cell_verts = L.cell_verts.p;
     v.vert_cell = L.vert_cell.p;
+    v.perm = M.d_perm.p;
     if (L.kind == 1) {
         L.adj_counts.alloc(n + 1);
         L.adj_start.alloc(n + 1);
@@ -393,7 +492,8 @@ struct tm_energy {
     int precision = TM_PREC_FP64;
     tm_nlist* nlist = nullptr;
     int nv = 0;
-    DevBuf<double> q, partials, grad, vprops;
+    DevBuf<double> q, partials, grad, vprops, gprops;
+    DevBuf<int> pair_cnt;
     DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<int> flip_result, proposals;
@@ -425,12 +525,29 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     if (E.nv == M.nv && E.q.p) return;
     E.nv = M.nv;
     E.q.alloc(M.nv);
+    E.gprops.alloc((size_t)M.nv * 6);
+    E.pair_cnt.alloc(M.nv);
+    HIPCHECK(hipMemsetAsync(E.pair_cnt.p, 0, sizeof(int) * M.nv, g_rt.stream));
     E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
     E.grad.alloc((size_t)M.nv * 3);
-    E.counter.alloc(1);
+    E.counter.alloc(2);
     E.globals.alloc(1);
     E.flip_result.alloc(4);
-    HIPCHECK(hipMemsetAsync(E.counter.p, 0, sizeof(unsigned), g_rt.stream));
+    HIPCHECK(hipMemsetAsync(E.counter.p, 0, 2 * sizeof(unsigned), g_rt.stream));
+}
+
+PassBuffers buffers(tm_energy& E, tm_mesh& M)
+{
+    PassBuffers b{};
+    b.q = E.q.p;
+    b.gprops = E.gprops.p;
+    b.pair_cnt = E.pair_cnt.p;
+    b.partials = E.partials.p;
+    b.counter = E.counter.p;
+    b.globals = E.globals.p;
+    b.list = M.generic.p;
+    b.nlist = M.n_generic;
+    return b;
 }
 
 void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
@@ -441,15 +558,14 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
         HIPCHECK(hipMemsetAsync(E.globals.p, 0, sizeof(Globals), g_rt.stream));
         return;
     }
-    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, E.q.p, vprops, E.partials.p, E.counter.p,
-                 E.globals.p, g_rt.stream);
+    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), vprops, g_rt.stream);
     check_launch();
 }
 
 void run_pass2(tm_energy& E, tm_mesh& M, int mixed)
 {
     if (M.nv == 0) return;
-    launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, E.q.p, E.globals.p, E.grad.p, g_rt.stream);
+    launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), E.grad.p, g_rt.stream);
     check_launch();
 }
 
@@ -533,6 +649,7 @@ int tm_mesh_create(const double* points, int n_points, const int* faces, int n_f
     }
     if (n_points) std::memcpy(M->points, points, sizeof(double) * 3 * n_points);
     build_topology(*M, faces, n_faces);
+    reorder_vertices(*M);
     *out = M.release();
     API_END
 }
@@ -560,11 +677,11 @@ int tm_mesh_fv_indices(tm_mesh* mesh, int* out)
     refresh_host_topology(*mesh);
     for (int f = 0; f < mesh->nf; f++) {
         int h = mesh->f_he[f];
-        out[3 * f] = mesh->he_to[h];
+        out[3 * f] = mesh->perm[mesh->he_to[h]];
         h = mesh->he_next[h];
-        out[3 * f + 1] = mesh->he_to[h];
+        out[3 * f + 1] = mesh->perm[mesh->he_to[h]];
         h = mesh->he_next[h];
-        out[3 * f + 2] = mesh->he_to[h];
+        out[3 * f + 2] = mesh->perm[mesh->he_to[h]];
     }
     API_END
 }
@@ -581,10 +698,12 @@ int tm_mesh_topology(tm_mesh* mesh, int* he_to, int* he_next, int* he_face, int*
 {
     API_BEGIN
     refresh_host_topology(*mesh);
-    if (he_to) std::memcpy(he_to, mesh->he_to.data(), sizeof(int) * mesh->nh);
+    if (he_to)
+        for (int h = 0; h < mesh->nh; h++) he_to[h] = mesh->perm[mesh->he_to[h]];
     if (he_next) std::memcpy(he_next, mesh->he_next.data(), sizeof(int) * mesh->nh);
     if (he_face) std::memcpy(he_face, mesh->he_face.data(), sizeof(int) * mesh->nh);
-    if (v_he) std::memcpy(v_he, mesh->v_he.data(), sizeof(int) * mesh->nv);
+    if (v_he)
+        for (int u = 0; u < mesh->nv; u++) v_he[u] = mesh->v_he[mesh->iperm[u]];
     if (f_he) std::memcpy(f_he, mesh->f_he.data(), sizeof(int) * mesh->nf);
     API_END
 }
@@ -612,7 +731,7 @@ int tm_mesh_download_points(tm_mesh* mesh)
 {
     API_BEGIN
     ensure_device(*mesh);
-    d2h(mesh->points, mesh->x.p, sizeof(double) * 3 * mesh->nv);
+    download_rows(*mesh, mesh->x.p, 3, mesh->points);
     sync();
     API_END
 }
@@ -714,7 +833,7 @@ int tm_energy_vertex_properties(tm_energy* em, tm_mesh* mesh, double* out)
     upload_points(*mesh);
     em->vprops.alloc((size_t)mesh->nv * 6);
     run_pass1(*em, *mesh, em->precision, em->vprops.p);
-    d2h(out, em->vprops.p, sizeof(double) * 6 * mesh->nv);
+    download_rows(*mesh, em->vprops.p, 6, out);
     sync();
     API_END
 }
@@ -740,7 +859,7 @@ int tm_energy_gradient(tm_energy* em, tm_mesh* mesh, double* out)
     upload_points(*mesh);
     run_pass1(*em, *mesh, em->precision, nullptr);
     run_pass2(*em, *mesh, em->precision);
-    d2h(out, em->grad.p, sizeof(double) * 3 * mesh->nv);
+    download_rows(*mesh, em->grad.p, 3, out);
     sync();
     API_END
 }
@@ -753,7 +872,7 @@ int tm_energy_energy_gradient(tm_energy* em, tm_mesh* mesh, double* energy, doub
     run_pass2(*em, *mesh, em->precision);
     Globals g;
     d2h(&g, em->globals.p, sizeof(Globals));
-    d2h(grad, em->grad.p, sizeof(double) * 3 * mesh->nv);
+    download_rows(*mesh, em->grad.p, 3, grad);
     sync();
     *energy = g.energy;
     API_END
@@ -865,6 +984,7 @@ int tm_nlist_point_distances(tm_nlist* nl, tm_mesh* mesh, int pid, double dmax, 
     require(pid >= 0 && pid < mesh->nv, "pid out of range");
     require(nl->nv == mesh->nv, "mesh does not match the neighbour list");
     upload_points(*mesh);
+    pid = mesh->iperm[pid];
     // first pass counts, second pass fills a buffer of that size
     nl->pd_count.alloc(1);
     launch_point_distances(mesh->view(), nl->view(), pid, dmax, 0, nl->pd_count.p, nullptr, nullptr, g_rt.stream);

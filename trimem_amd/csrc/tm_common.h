@@ -96,6 +96,7 @@ struct NListView {
     const int* __restrict__ cell_start;    // [ncells + 1]
     const int* __restrict__ cell_verts;    // vertices sorted by (cell, id)
     const int* __restrict__ vert_cell;     // build-time cell of each vertex
+    const int* __restrict__ perm;          // internal -> user vertex id
     // verlet list
     const int* __restrict__ adj_start;     // [N + 1]
     const int* __restrict__ adj;           // neighbour ids, ascending per vertex

@@ -68,6 +68,19 @@ __device__ __forceinline__ d3 load_p(const double* __restrict__ x, int v)
     return d3{x[3 * v], x[3 * v + 1], x[3 * v + 2]};
 }
 
+// ------------------------------------------------------- block mapping ---
+// Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD
+// b % 8), each with its own L2.  Launching a multiple of 8 blocks and giving
+// XCD x the logical blocks [x * G/8, (x+1) * G/8) makes each XCD work on one
+// contiguous range of (spatially ordered) vertices, so its L2 holds that
+// region and its halo instead of a slice of the whole mesh.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int logical_block()
+{
+    const int chunk = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+}
+
 // ------------------------------------------------------------ reduction ---
 // Sum six per-thread doubles over the block; result valid in s_out[0..5].
 __device__ __forceinline__ void block_sum6(double v[6], double* s_red /* [nwaves*6] */, double* s_out)
@@ -125,75 +138,219 @@ __device__ void finalize_globals(const double* __restrict__ partials, int nblock
     }
 }
 
+// ------------------------------------------------------- lane groups -----
+// Each vertex is handled by a group of kGroup lanes: lane k of the group owns
+// ring slot k (and k + kGroup, ... for valence above kGroup).  All loads of a
+// vertex's ring are issued at once instead of as a chain of dependent
+// round trips, and the group sums its per-slot terms with lane shuffles.
+constexpr int kGroup = 8;
+
+__device__ __forceinline__ double group_sum(double x)
+{
+    x += __shfl_xor(x, 4, kGroup);
+    x += __shfl_xor(x, 2, kGroup);
+    x += __shfl_xor(x, 1, kGroup);
+    return x;
+}
+__device__ __forceinline__ int group_sum(int x)
+{
+    x += __shfl_xor(x, 4, kGroup);
+    x += __shfl_xor(x, 2, kGroup);
+    x += __shfl_xor(x, 1, kGroup);
+    return x;
+}
+
+// Exclusion test of the cell list (cell_list.h:170-202) on the ring table:
+// j is excluded at level 1 if it is in ring(v), and at level 2 also if ring(j)
+// and ring(v) intersect (j is then a neighbour of a neighbour of v).
+__device__ __forceinline__ bool ring_excluded(const MeshView& m, int v, int n, int j, int excl)
+{
+    if (j == v) return true;
+    if (excl <= 0) return false;
+    const int N = m.nv;
+    for (int k = 0; k < n; k++)
+        if (m.ring[k * N + v] == j) return true;
+    if (excl == 1) return false;
+    const int nj = meta_valence(m.meta[j]);
+    for (int a = 0; a < nj; a++) {
+        int w = m.ring[a * N + j];
+        for (int k = 0; k < n; k++)
+            if (m.ring[k * N + v] == w) return true;
+    }
+    return false;
+}
+
+// One vertex's ring held in registers for exclusion tests: slots beyond the
+// valence hold -1.  Rings longer than kGroup fall back to ring_excluded.
+struct RingRegs {
+    int r[kGroup];
+    int n;
+};
+__device__ __forceinline__ RingRegs load_ring(const MeshView& m, int v, int n)
+{
+    RingRegs R;
+    R.n = n;
+#pragma unroll
+    for (int k = 0; k < kGroup; k++) R.r[k] = k < n ? m.ring[k * m.nv + v] : -1;
+    return R;
+}
+__device__ __forceinline__ bool in_ring(const RingRegs& R, int j)
+{
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < kGroup; k++) hit |= R.r[k] == j;
+    return hit;
+}
+// Same answer as ring_excluded(m, v, n, j, excl) for n <= kGroup, with the
+// ring of v in registers and the ring of j fetched as independent loads.
+__device__ __forceinline__ bool excluded_fast(const MeshView& m, const RingRegs& R, int v, int j, int excl)
+{
+    if (j == v) return true;
+    if (excl <= 0) return false;
+    if (R.n > kGroup) return ring_excluded(m, v, R.n, j, excl);
+    if (in_ring(R, j)) return true;
+    if (excl == 1) return false;
+    const int nj = meta_valence(m.meta[j]);
+    int w[kGroup];
+#pragma unroll
+    for (int a = 0; a < kGroup; a++) w[a] = a < nj ? m.ring[a * m.nv + j] : -2;
+    bool hit = false;
+#pragma unroll
+    for (int a = 0; a < kGroup; a++) hit |= in_ring(R, w[a]);
+    if (hit) return true;
+    for (int a = kGroup; a < nj; a++)
+        if (in_ring(R, m.ring[a * m.nv + j])) return true;
+    return false;
+}
+
+// The neighbour query of point_distances (cell_list.h:133-219,
+// verlet_list.h:76-101) shared by the lanes of a group: lane r scans cell
+// row r of the 3x3 rows of the stencil (lane 0 also row 8), two candidates
+// per step; for the Verlet list the adjacency is strided over the lanes.
+// f(j, d) sees the pairs of this lane; together the lanes see the
+// reference's set exactly once.
+template <class F>
+__device__ __forceinline__ void group_neighbours(const MeshView& m, const NListView& nl, int v, int n, d3 pi,
+                                                 double dmax, int lane, F&& f)
+{
+    const double dmax2 = dmax * dmax * (1.0 + 1e-15);
+    const RingRegs R = load_ring(m, v, n);
+    auto test = [&](int j, d3 pj) {
+        d3 d = pi - pj;
+        double sq = sqnorm(d);
+        if (sq <= dmax2 && sqrt(sq) < dmax) {
+            if (nl.kind == 1 || !excluded_fast(m, R, v, j, nl.exclusion)) f(j, d);
+        }
+    };
+    auto scan = [&](const int* ids, int s, int e, int step) {
+        for (int q = s; q < e; q += 2 * step) {
+            int j0 = ids[q];
+            bool two = q + step < e;
+            int j1 = two ? ids[q + step] : j0;
+            d3 p0 = load_p(m.x, j0);
+            d3 p1 = load_p(m.x, j1);
+            test(j0, p0);
+            if (two) test(j1, p1);
+        }
+    };
+    if (nl.kind == 0) {
+        int c[3];
+        cell_coords(nl, pi, c);
+        const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
+        if (i0 > i1) return;
+        for (int r = lane; r < 9; r += kGroup) {
+            int kk = c[2] + r / 3 - 1;
+            int jj = c[1] + r % 3 - 1;
+            if (kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) continue;
+            int base = jj * nl.strides[1] + kk * nl.strides[2];
+            scan(nl.cell_verts, nl.cell_start[base + i0], nl.cell_start[base + i1 + 1], 1);
+        }
+    } else if (nl.kind == 1) {
+        scan(nl.adj, nl.adj_start[v] + lane, nl.adj_start[v + 1], kGroup);
+    }
+}
+
 // --------------------------------------------------------------- pass 1 ---
 // Ring slot k of an interior vertex v: halfedge v -> j_k, its face
 // F_k = (v, j_k, j_{k-1}) and the twin face (j_k, v, j_{k+1}) = F_{k+1}.
 // With e_k = x_{j_k} - x_v, the OpenMesh sector normal of v -> j_k is
 // n_k = e_k x e_{k-1} (|n_k| = 2 area(F_k)) and that of j_k -> v is n_{k+1}.
+// One slot contributes area(F_k), vol(F_k) = x_v . n_k / 6, l_k theta_k / 2
+// and the tether term of halfedge v -> j_k (mesh_properties.cpp:19-33).
 template <class T>
-__device__ __forceinline__ void ring_properties(const MeshView& m, const EParams& p, int v, int n,
-                                                double out[6])
+__device__ __forceinline__ void slot_properties(const MeshView& m, const EParams& p, int v, int n, int k, d3 xv,
+                                                double& area, double& vol, double& curv, double& teth)
 {
     const int N = m.nv;
-    const d3 xv = load_p(m.x, v);
-    d3 e_prev = load_p(m.x, m.ring[(n - 1) * N + v]) - xv;
-    d3 e_cur = load_p(m.x, m.ring[v]) - xv;
-    d3 n_cur = cross(e_cur, e_prev);
-    double area = 0, vol = 0, curv = 0, teth = 0;
-    for (int k = 0; k < n; k++) {
-        int kn = (k + 1 == n) ? 0 : k + 1;
-        d3 e_next = load_p(m.x, m.ring[kn * N + v]) - xv;
-        d3 n_next = cross(e_next, e_cur);
-        T af = Prec<T>::sqrt_(sqnorm(n_cur)) * (T)0.5;
-        T l = Prec<T>::sqrt_(sqnorm(e_cur));
-        T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
-        curv += (double)((T)0.5 * l * th);
-        area += (double)af;
-        vol += dot(xv, n_cur) / 6;    // = x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
-        if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
-        else teth += bond_area_value((double)af, p);
-        e_prev = e_cur;
-        e_cur = e_next;
-        n_cur = n_next;
-    }
-    double a = area / 3, c = curv / 2;
-    out[0] = a;
-    out[1] = vol / 3;
-    out[2] = c;
-    out[3] = 2 * c * c / a;
-    out[4] = teth / (p.bond_type == 0 ? 2 : 3);
+    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
+    const d3 e_prev = load_p(m.x, m.ring[km * N + v]) - xv;
+    const d3 e_cur = load_p(m.x, m.ring[k * N + v]) - xv;
+    const d3 e_next = load_p(m.x, m.ring[kp * N + v]) - xv;
+    const d3 n_cur = cross(e_cur, e_prev);
+    const d3 n_next = cross(e_next, e_cur);
+    T af = Prec<T>::sqrt_(sqnorm(n_cur)) * (T)0.5;
+    T l = Prec<T>::sqrt_(sqnorm(e_cur));
+    T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
+    curv += (double)((T)0.5 * l * th);
+    area += (double)af;
+    vol += dot(xv, n_cur) / 6;    // = x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
+    if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
+    else teth += bond_area_value((double)af, p);
 }
 
-__device__ __forceinline__ double vertex_repulsion(const MeshView& m, const NListView& nl, const EParams& p, int v)
+__device__ __forceinline__ bool is_generic(int meta)
 {
-    if (!(p.rep_lc1 > 0) || nl.kind < 0) return 0.0;
-    const Topo t = topo_of(m);
-    const d3 pi = load_p(m.x, v);
-    double rep = 0;
-    auto ex = [&](int a, int b) { return nl.kind == 0 && om_excluded(t, a, b, nl.exclusion); };
-    nlist_visit(nl, m.x, v, pi, p.rep_lc1, ex, [&](int, d3 d) { rep += rep_value(norm(d), p.rep_lc1, p.rep_r); });
-    return rep;
+    return meta_bslot(meta) >= 0 || meta_valence(meta) < 3;
 }
 
+// Pass 1 over all vertices; vertices on the boundary (or with fewer than
+// three neighbours) read the properties that k_generic_props wrote.  Writes
+// q_v, the number of repulsion pairs of v, and the FP64 globals.
 template <class T>
 __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams p, double* __restrict__ q,
-                                               double* __restrict__ vprops, double* __restrict__ partials,
-                                               unsigned* __restrict__ counter, Globals* __restrict__ g)
+                                               double* __restrict__ vprops, const double* __restrict__ gprops,
+                                               int* __restrict__ pair_cnt, double* __restrict__ partials)
 {
     __shared__ double s_red[4 * 6];
     __shared__ double s_out[6];
-    __shared__ int s_last;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const int v = t / kGroup, lane = t % kGroup;
+    const bool active = v < m.nv;
+    const int meta = active ? m.meta[v] : 0;
+    const int n = meta_valence(meta);
+    const bool ring = active && !is_generic(meta);
+    const bool rep_on = ring && p.rep_lc1 > 0 && nl.kind >= 0;
+    double area = 0, vol = 0, curv = 0, teth = 0, rep = 0;
+    int cnt = 0;
+    if (ring) {
+        const d3 xv = load_p(m.x, v);
+        for (int k = lane; k < n; k += kGroup) slot_properties<T>(m, p, v, n, k, xv, area, vol, curv, teth);
+        if (rep_on) {
+            group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, [&](int, d3 d) {
+                rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
+                cnt++;
+            });
+        }
+    }
+    area = group_sum(area);
+    vol = group_sum(vol);
+    curv = group_sum(curv);
+    teth = group_sum(teth);
+    rep = group_sum(rep);
+    cnt = group_sum(cnt);
     double pr[6] = {0, 0, 0, 0, 0, 0};
-    if (v < m.nv) {
-        const int meta = m.meta[v];
-        const int n = meta_valence(meta);
-        if (meta_bslot(meta) < 0 && n >= 3) {
-            ring_properties<T>(m, p, v, n, pr);
-            pr[5] = vertex_repulsion(m, nl, p, v);
+    if (active && lane == 0) {
+        if (ring) {
+            double a = area / 3, c = curv / 2;
+            pr[0] = a;
+            pr[1] = vol / 3;
+            pr[2] = c;
+            pr[3] = 2 * c * c / a;
+            pr[4] = teth / (p.bond_type == 0 ? 2 : 3);
+            pr[5] = rep;
+            pair_cnt[v] = cnt;
         } else {
-            VP r = om_vertex_properties(topo_of(m), nl, p, v);
-            for (int k = 0; k < 6; k++) pr[k] = r.v[k];
+            for (int k = 0; k < 6; k++) pr[k] = gprops[6 * v + k];
         }
         q[v] = pr[2] / pr[0];
         if (vprops) {
@@ -202,15 +359,43 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
     }
     block_sum6(pr, s_red, s_out);
     if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = s_out[threadIdx.x];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        finalize_globals(partials, gridDim.x, p, g, s_red, s_out);
-        if (threadIdx.x == 0) *counter = 0u;
-    }
+}
+
+// Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
+// factors (one block; kernel boundaries order it after pass 1).
+__global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, EParams p,
+                                                   Globals* __restrict__ g)
+{
+    __shared__ double s_red[16 * 6];
+    __shared__ double s_out[6];
+    finalize_globals(partials, nblocks, p, g, s_red, s_out);
+}
+
+// OpenMesh-semantics properties of the listed vertices (boundary and
+// degenerate rings), FP64.
+__global__ void __launch_bounds__(128) k_generic_props(MeshView m, NListView nl, EParams p,
+                                                       const int* __restrict__ list, int nlist,
+                                                       double* __restrict__ gprops)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlist) return;
+    int v = list[i];
+    VP r = om_vertex_properties(topo_of(m), nl, p, v);
+    for (int k = 0; k < 6; k++) gprops[6 * v + k] = r.v[k];
+}
+
+__global__ void __launch_bounds__(128) k_generic_grad(MeshView m, NListView nl, EParams p,
+                                                      const double* __restrict__ q, const Globals* __restrict__ gp,
+                                                      const int* __restrict__ list, int nlist,
+                                                      double* __restrict__ grad)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlist) return;
+    int v = list[i];
+    d3 out = om_vertex_gradient(topo_of(m), nl, p, *gp, q, v);
+    grad[3 * v] = out.x;
+    grad[3 * v + 1] = out.y;
+    grad[3 * v + 2] = out.z;
 }
 
 // --------------------------------------------------------------- pass 2 ---
@@ -231,109 +416,111 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
 //  * The theta * dl parts of the spokes stay as they are: -(q_v + q_k) theta e_hat.
 //  * The face-volume gradients cross(x_j, x_k)/6 sum to sum_k n_k / 6 around a
 //    closed ring (the x_v x rim terms telescope to zero).
+// The penalty factors are known when pass 2 runs, so every contribution goes
+// straight into one accumulator.
 template <class T>
-__device__ __forceinline__ d3 ring_gradient(const MeshView& m, const NListView& nl, const EParams& p,
-                                            const Globals& g, const double* __restrict__ q, int v, int n)
+__device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p, const Globals& g,
+                                            const double* __restrict__ q, int v, int n, int k, d3 xv, double qv)
 {
     const int N = m.nv;
-    const d3 xv = load_p(m.x, v);
-    const double qv = q[v];
-    int j = m.ring[(n - 1) * N + v];
-    d3 e_prev = load_p(m.x, j) - xv;
-    double q_prev = q[j];
-    j = m.ring[v];
-    d3 e_cur = load_p(m.x, j) - xv;
-    double q_cur = q[j];
-    d3 n_cur = cross(e_cur, e_prev);
-    d3 gA{0, 0, 0}, gN{0, 0, 0}, gB{0, 0, 0}, gC{0, 0, 0}, gT{0, 0, 0};
-    const double kb = p.kappa_b;
-    for (int k = 0; k < n; k++) {
-        int kn = (k + 1 == n) ? 0 : k + 1;
-        j = m.ring[kn * N + v];
-        d3 e_next = load_p(m.x, j) - xv;
-        double q_next = q[j];
-        d3 n_next = cross(e_next, e_cur);
-        // face F_k = (v, j_k, j_{k-1})
-        T n0l = Prec<T>::sqrt_(sqnorm(n_cur));
-        T inv0 = (T)1 / n0l;
-        d3 rim = e_prev - e_cur;                               // x_{k-1} - x_k
-        T cot_j = (T)(-dot(rim, e_cur)) * inv0;                // angle at j_k
-        T cot_i = (T)dot(e_prev, rim) * inv0;                  // angle at j_{k-1}
-        d3 u = (double)inv0 * n_cur;
-        d3 dA = 0.5 * cross(u, rim);                           // d area(F_k) / d x_v
-        gA += dA;
-        gN += n_cur;
-        double wq = (double)cot_j * (qv - q_prev) + (double)cot_i * (qv - q_cur);
-        double q2 = qv * qv + q_cur * q_cur + q_prev * q_prev;
-        gB += wq * u;
-        gB -= (2.0 / 3.0 * q2) * dA;
-        // spoke (v, j_k) with faces F_k and F_{k+1}
-        T l = Prec<T>::sqrt_(sqnorm(e_cur));
-        T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
-        d3 eh = (double)((T)1 / l) * e_cur;
-        gB -= ((qv + q_cur) * (double)th) * eh;
-        gC -= (0.5 * (double)th) * eh;
-        if (p.bond_type == 0) gT -= bond_edge_dfac((double)l, p) * eh;
-        else gT += bond_area_dfac((double)(n0l * (T)0.5), p) * dA;
-        e_prev = e_cur;
-        q_prev = q_cur;
-        e_cur = e_next;
-        q_cur = q_next;
-        n_cur = n_next;
-    }
-    d3 gR{0, 0, 0};
-    if (p.rep_lc1 > 0 && nl.kind >= 0 && p.kappa_r != 0) {
-        const Topo t = topo_of(m);
-        auto ex = [&](int a, int b) { return nl.kind == 0 && om_excluded(t, a, b, nl.exclusion); };
-        nlist_visit(nl, m.x, v, xv, p.rep_lc1, ex,
-                    [&](int, d3 d) { gR += (2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d; });
-    }
-    d3 out{0, 0, 0};
-    out += g.fac_a * gA;
-    out += (g.fac_v / 6.0) * gN;
-    out += g.fac_c * gC;
-    out += p.kappa_t * gT;
-    out += p.kappa_r * gR;
-    out += kb * gB;
-    return out;
+    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
+    const int jm = m.ring[km * N + v], jc = m.ring[k * N + v], jp = m.ring[kp * N + v];
+    const d3 e_prev = load_p(m.x, jm) - xv;
+    const d3 e_cur = load_p(m.x, jc) - xv;
+    const d3 e_next = load_p(m.x, jp) - xv;
+    const double q_prev = q[jm], q_cur = q[jc];
+    const double kb = p.kappa_b, kt = p.kappa_t;
+    const d3 n_cur = cross(e_cur, e_prev);
+    const d3 n_next = cross(e_next, e_cur);
+    // face F_k = (v, j_k, j_{k-1})
+    T n0l = Prec<T>::sqrt_(sqnorm(n_cur));
+    T inv0 = (T)1 / n0l;
+    d3 rim = e_prev - e_cur;                               // x_{k-1} - x_k
+    T cot_j = (T)(-dot(rim, e_cur)) * inv0;                // angle at j_k
+    T cot_i = (T)dot(e_prev, rim) * inv0;                  // angle at j_{k-1}
+    d3 u = (double)inv0 * n_cur;
+    d3 dA = 0.5 * cross(u, rim);                           // d area(F_k) / d x_v
+    double wq = (double)cot_j * (qv - q_prev) + (double)cot_i * (qv - q_cur);
+    double q2 = qv * qv + q_cur * q_cur + q_prev * q_prev;
+    double fdA = g.fac_a - kb * (2.0 / 3.0) * q2;
+    if (p.bond_type == 1) fdA += kt * bond_area_dfac((double)(n0l * (T)0.5), p);
+    d3 acc = fdA * dA;
+    acc += (g.fac_v / 6.0) * n_cur;
+    acc += (kb * wq) * u;
+    // spoke (v, j_k) with faces F_k and F_{k+1}
+    T l = Prec<T>::sqrt_(sqnorm(e_cur));
+    T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
+    double fe = -(kb * (qv + q_cur) + 0.5 * g.fac_c) * (double)th;
+    if (p.bond_type == 0) fe -= kt * bond_edge_dfac((double)l, p);
+    acc += (fe * (double)((T)1 / l)) * e_cur;
+    return acc;
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_pass2(MeshView m, NListView nl, EParams p, const double* __restrict__ q,
-                                               const Globals* __restrict__ gp, double* __restrict__ grad)
+                                               const Globals* __restrict__ gp, const int* __restrict__ pair_cnt,
+                                               double* __restrict__ grad)
 {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= m.nv) return;
-    const Globals g = *gp;
-    const int meta = m.meta[v];
+    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const int v = t / kGroup, lane = t % kGroup;
+    const bool active = v < m.nv;
+    const int meta = active ? m.meta[v] : 0;
     const int n = meta_valence(meta);
-    d3 out;
-    if (meta_bslot(meta) < 0 && n >= 3) out = ring_gradient<T>(m, nl, p, g, q, v, n);
-    else out = om_vertex_gradient(topo_of(m), nl, p, g, q, v);
-    grad[3 * v] = out.x;
-    grad[3 * v + 1] = out.y;
-    grad[3 * v + 2] = out.z;
+    const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
+    d3 acc{0, 0, 0};
+    if (ring) {
+        const Globals g = *gp;
+        const d3 xv = load_p(m.x, v);
+        const double qv = q[v];
+        for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, q, v, n, k, xv, qv);
+        // repulsion (mesh_repulsion.h:56-79, doubled as in
+        // mesh_properties.cpp:138-146) for the vertices pass 1 found pairs for
+        if (p.rep_lc1 > 0 && nl.kind >= 0 && p.kappa_r != 0 && pair_cnt[v] > 0) {
+            group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, [&](int, d3 d) {
+                acc += (p.kappa_r * 2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d;
+            });
+        }
+    }
+    acc.x = group_sum(acc.x);
+    acc.y = group_sum(acc.y);
+    acc.z = group_sum(acc.z);
+    if (ring && lane == 0) {
+        grad[3 * v] = acc.x;
+        grad[3 * v + 1] = acc.y;
+        grad[3 * v + 2] = acc.z;
+    }
 }
 
 // ------------------------------------------------------------ launchers ---
 static inline int nblocks(int n, int b) { return (n + b - 1) / b; }
 
-int pass1_blocks(int nv) { return nblocks(nv, 256); }
+int pass1_blocks(int nv) { return nblocks(nblocks(nv * kGroup, 256), kXcds) * kXcds; }
 
-void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, double* q, double* vprops,
-                  double* partials, unsigned* counter, Globals* g, hipStream_t s)
+void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
+                  double* vprops, hipStream_t s)
 {
-    dim3 grid(nblocks(m.nv, 256)), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, m, nl, p, q, vprops, partials, counter, g);
-    else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, m, nl, p, q, vprops, partials, counter, g);
+    if (b.nlist > 0)
+        hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
+                           b.gprops);
+    dim3 grid(pass1_blocks(m.nv)), block(256);
+    if (mixed)
+        hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, b.pair_cnt,
+                           b.partials);
+    else
+        hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, b.pair_cnt,
+                           b.partials);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, b.partials, (int)grid.x, p, b.globals);
 }
 
-void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const double* q,
-                  const Globals* g, double* grad, hipStream_t s)
+void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
+                  double* grad, hipStream_t s)
 {
-    dim3 grid(nblocks(m.nv, 256)), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass2<float>, grid, block, 0, s, m, nl, p, q, g, grad);
-    else hipLaunchKernelGGL(k_pass2<double>, grid, block, 0, s, m, nl, p, q, g, grad);
+    dim3 grid(pass1_blocks(m.nv)), block(256);
+    if (mixed) hipLaunchKernelGGL(k_pass2<float>, grid, block, 0, s, m, nl, p, b.q, b.globals, b.pair_cnt, grad);
+    else hipLaunchKernelGGL(k_pass2<double>, grid, block, 0, s, m, nl, p, b.q, b.globals, b.pair_cnt, grad);
+    if (b.nlist > 0)
+        hipLaunchKernelGGL(k_generic_grad, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.q, b.globals,
+                           b.list, b.nlist, grad);
 }
 
 // --------------------------------------------- single-halfedge geometry ---

@@ -49,6 +49,22 @@ void launch_build_ring(const MeshView& m, int* ring, int* meta, int* overflow, h
     hipLaunchKernelGGL(k_build_ring, dim3((m.nv + 255) / 256), dim3(256), 0, s, m, ring, meta, overflow);
 }
 
+// ------------------------------------------------------- permutations -----
+__global__ void k_gather_rows(const double* __restrict__ src, const int* __restrict__ idx, double* __restrict__ dst,
+                              int n, int w)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * w) return;
+    int i = t / w, c = t - i * w;
+    dst[t] = src[(size_t)idx[i] * w + c];
+}
+
+void launch_gather_rows(const double* src, const int* idx, double* dst, int n, int w, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3((n * w + 255) / 256), dim3(256), 0, s, src, idx, dst, n, w);
+}
+
 // ---------------------------------------------------------------- RNG -----
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
 {

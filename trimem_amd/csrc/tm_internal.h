@@ -6,28 +6,43 @@
 namespace tmh {
 
 // energy passes (tm_energy.hip)
+struct PassBuffers {
+    double* q;          // [N] curvature / area per vertex
+    double* gprops;     // [6N] properties of listed (boundary) vertices
+    int* pair_cnt;      // [N] repulsion pairs found by pass 1
+    double* partials;   // [6 * pass1 blocks]
+    unsigned* counter;  // last-block counter of pass 1 (returns to 0)
+    Globals* globals;
+    const int* list;    // boundary / degenerate vertices
+    int nlist;
+};
 int pass1_blocks(int nv);
-void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, double* q, double* vprops,
-                  double* partials, unsigned* counter, Globals* g, hipStream_t s);
-void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const double* q,
-                  const Globals* g, double* grad, hipStream_t s);
+void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
+                  double* vprops, hipStream_t s);
+void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
+                  double* grad, hipStream_t s);
 void launch_halfedge_geometry(const MeshView& m, int h, double* out, hipStream_t s);
 void launch_sum_geometry(const MeshView& m, int what, double* partials, unsigned* counter, double* result,
                          hipStream_t s);
 
-// topology helpers (tm_mesh_kernels.hip)
+// dst[i*w + c] = src[idx[i]*w + c] for i < n, c < w (tm_flips.hip)
+void launch_gather_rows(const double* src, const int* idx, double* dst, int n, int w, hipStream_t s);
+
+// topology helpers (tm_flips.hip)
 // Rebuild ring table and meta from the halfedge arrays; sets *overflow to 1
 // if a vertex has more than kMaxValence neighbours.
 void launch_build_ring(const MeshView& m, int* ring, int* meta, int* overflow, hipStream_t s);
 
 // neighbour lists (tm_nlist.hip)
-void launch_box(const double* x, int n, double* box6, hipStream_t s);
+int box_scratch_doubles();
+void launch_box(const double* x, int n, double* scratch, unsigned* counter, double* box6, hipStream_t s);
 void launch_cell_ids(const double* x, int n, const NListView& nl, int* vert_cell, int* counts, hipStream_t s);
 void launch_exclusive_scan(const int* in, int* out, int n, int* block_sums, hipStream_t s);
 int scan_block_sums_size(int n);
 void launch_cell_scatter(const int* vert_cell, int n, const int* cell_start, int* fill, int* cell_verts,
                          hipStream_t s);
-void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, int* cell_verts, hipStream_t s);
+void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, const int* perm, int* cell_verts,
+                               hipStream_t s);
 // distance_matrix (cell list or verlet) in two phases: counts then fill
 void launch_dm_count(const MeshView& m, const NListView& nl, double dmax, int* counts, hipStream_t s);
 void launch_dm_fill(const MeshView& m, const NListView& nl, double dmax, const int* offsets, double* d, int* i,

@@ -25,12 +25,18 @@
 namespace tmh {
 
 // ----------------------------------------------------------------- box ----
-__global__ void __launch_bounds__(1024) k_box(const double* __restrict__ x, int n, double* __restrict__ box)
+// Bounding box with the reference's initial values (cell_list.h:46-47):
+// min = (+DBL_MAX, 0, 0), max = (-1, 0, 0).  Grid-stride partial min/max per
+// block, and the last block to finish combines the partials.
+constexpr int kBoxBlocks = 64;
+__global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n, double* __restrict__ part,
+                                              unsigned* __restrict__ counter, double* __restrict__ box)
 {
-    __shared__ double s[6][16];
+    __shared__ double s[6][4];
+    __shared__ int s_last;
     double mn[3] = {DBL_MAX, 0.0, 0.0};
     double mx[3] = {-1.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         for (int k = 0; k < 3; k++) {
             double v = x[3 * i + k];
             if (v < mn[k]) mn[k] = v;
@@ -54,14 +60,29 @@ __global__ void __launch_bounds__(1024) k_box(const double* __restrict__ x, int 
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
         double r = s[k][0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
+        for (int w = 1; w < 4; w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
+        part[blockIdx.x * 6 + k] = r;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        double r = part[k];
+        for (int b = 1; b < (int)gridDim.x; b++) r = k < 3 ? fmin(r, part[b * 6 + k]) : fmax(r, part[b * 6 + k]);
         box[k] = r;
     }
+    if (threadIdx.x == 0) *counter = 0u;
 }
 
-void launch_box(const double* x, int n, double* box6, hipStream_t s)
+int box_scratch_doubles() { return kBoxBlocks * 6 + 8; }
+
+void launch_box(const double* x, int n, double* scratch, unsigned* counter, double* box6, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_box, dim3(1), dim3(1024), 0, s, x, n, box6);
+    hipLaunchKernelGGL(k_box, dim3(kBoxBlocks), dim3(256), 0, s, x, n, scratch, counter, box6);
 }
 
 // ------------------------------------------------------------- binning ----
@@ -112,9 +133,10 @@ void launch_cell_scatter(const int* vert_cell, int n, const int* cell_start, int
 }
 
 // The thread sitting at the start of a cell's segment sorts the segment by
-// vertex id (cells hold a handful of vertices).
+// user vertex id (cells hold a handful of vertices), the order in which the
+// reference fills its cells.
 __global__ void k_cell_sort_segments(const int* __restrict__ vert_cell, int n, const int* __restrict__ cell_start,
-                                     int* __restrict__ cell_verts)
+                                     const int* __restrict__ perm, int* __restrict__ cell_verts)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -123,8 +145,9 @@ __global__ void k_cell_sort_segments(const int* __restrict__ vert_cell, int n, c
     int e = cell_start[c + 1];
     for (int a = i + 1; a < e; a++) {
         int key = cell_verts[a];
+        int kp = perm[key];
         int b = a - 1;
-        while (b >= i && cell_verts[b] > key) {
+        while (b >= i && perm[cell_verts[b]] > kp) {
             cell_verts[b + 1] = cell_verts[b];
             b--;
         }
@@ -132,9 +155,10 @@ __global__ void k_cell_sort_segments(const int* __restrict__ vert_cell, int n, c
     }
 }
 
-void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, int* cell_verts, hipStream_t s)
+void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, const int* perm, int* cell_verts,
+                               hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cell_sort_segments, dim3((n + 255) / 256), dim3(256), 0, s, vert_cell, n, cell_start,
+    hipLaunchKernelGGL(k_cell_sort_segments, dim3((n + 255) / 256), dim3(256), 0, s, vert_cell, n, cell_start, perm,
                        cell_verts);
 }
 
@@ -294,14 +318,16 @@ __global__ void k_dm_fill(MeshView m, NListView nl, double dmax, const int* __re
     if (a >= m.nv) return;
     int o = offsets[a];
     const bool ordered = nl.kind == 0;
+    const int ua = nl.perm[a];
     dm_visit(m, nl, dmax, a, [&](int b, double dd) {
+        const int ub = nl.perm[b];
         d[o] = dd;
         if (ordered) {
-            ii[o] = a < b ? a : b;
-            jj[o] = a < b ? b : a;
+            ii[o] = ua < ub ? ua : ub;
+            jj[o] = ua < ub ? ub : ua;
         } else {
-            ii[o] = a;
-            jj[o] = b;
+            ii[o] = ua;
+            jj[o] = ub;
         }
         o++;
     });
@@ -336,7 +362,7 @@ __global__ void k_point_distances(MeshView m, NListView nl, int pid, double dmax
                 dist3[3 * c] = d.x;
                 dist3[3 * c + 1] = d.y;
                 dist3[3 * c + 2] = d.z;
-                jdx[c] = j;
+                jdx[c] = nl.perm[j];
             }
             c++;
         }

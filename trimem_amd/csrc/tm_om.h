@@ -164,13 +164,22 @@ __device__ __forceinline__ d3 mu_dihedral_angle_grad(const Topo& t, int h, int b
     return -(1. / tan(a02) + 1. / tan(a04)) * n1 / l;
 }
 
+// x^r for the integer exponents of the potentials (std::pow with an int
+// exponent in the reference); exact for the small powers used in practice.
+__device__ __forceinline__ double ipow(double x, int r)
+{
+    double y = 1.0;
+    for (int i = 0; i < r; i++) y *= x;
+    return y;
+}
+
 // ------------------------------------------------- bond potentials --------
 // mesh_tether.h:40-55 (edge) and 94-102 (area), per halfedge
 __device__ __forceinline__ double bond_edge_value(double l, const EParams& p)
 {
     int r = p.bond_r;
-    if (l > p.bond_lc0) return pow((double)r, (double)(r + 1)) * pow(l - p.bond_lc0, (double)r);
-    if (l < p.bond_lc1) return exp(l / (l - p.bond_lc1)) / pow(l, (double)r);
+    if (l > p.bond_lc0) return ipow((double)r, r + 1) * ipow(l - p.bond_lc0, r);
+    if (l < p.bond_lc1) return exp(l / (l - p.bond_lc1)) / ipow(l, r);
     return 0.0;
 }
 // derivative factor dV/dl (mesh_tether.h:58-82; the two branches accumulate)
@@ -178,9 +187,9 @@ __device__ __forceinline__ double bond_edge_dfac(double l, const EParams& p)
 {
     int r = p.bond_r;
     double fac = 0.0;
-    if (l > p.bond_lc0) fac = pow((double)r, (double)(r + 2)) * pow(l - p.bond_lc0, (double)(r - 1));
+    if (l > p.bond_lc0) fac = ipow((double)r, r + 2) * ipow(l - p.bond_lc0, r - 1);
     if (l < p.bond_lc1) {
-        double lr = pow(l, (double)r);
+        double lr = ipow(l, r);
         double lmc = l - p.bond_lc1;
         fac -= p.bond_lc1 / (lmc * lmc) + r / l;
         fac /= lr;
@@ -202,12 +211,12 @@ __device__ __forceinline__ double bond_area_dfac(double a, const EParams& p)
 // mesh_repulsion.h:38-54 potential and 56-79 gradient factor (per pair)
 __device__ __forceinline__ double rep_value(double l, double lc1, int r)
 {
-    return exp(l / (l - lc1)) / pow(l, (double)r);
+    return exp(l / (l - lc1)) / ipow(l, r);
 }
 __device__ __forceinline__ double rep_dfac(double l, double lc1, int r)
 {
     double fac = 0.0;
-    double lr = pow(l, (double)(r + 1));
+    double lr = ipow(l, r + 1);
     double lmc = l - lc1;
     fac -= lc1 / (lmc * lmc) + r / l;
     fac /= lr;
@@ -253,11 +262,16 @@ __device__ __forceinline__ void cell_coords(const NListView& nl, d3 p, int c[3])
 // Visit every neighbour j of vertex i within dmax (point_distances of
 // cell_list.h:133-219 / verlet_list.h:76-101), as a set.  The exclusion
 // test is applied only to candidates inside dmax, which leaves the set
-// unchanged and skips most ring walks.  f(j, d) gets d = x_i - x_j.
+// unchanged and skips most ring walks.  The distance test is the
+// reference's norm(x_i - x_j) < dmax, evaluated exactly in FP64.  f(j, d)
+// gets d = x_i - x_j.
 template <class Excl, class F>
 __device__ __forceinline__ void nlist_visit(const NListView& nl, const double* __restrict__ x, int i, d3 pi,
                                             double dmax, Excl&& excluded, F&& f)
 {
+    // squared-distance prefilter; every pair with sqrt(s) < dmax (correctly
+    // rounded) has s below dmax^2 (1 + 1e-15), so only the exact test decides
+    const double dmax2 = dmax * dmax * (1.0 + 1e-15);
     if (nl.kind == 0) {
         int c[3];
         cell_coords(nl, pi, c);
@@ -277,7 +291,8 @@ __device__ __forceinline__ void nlist_visit(const NListView& nl, const double* _
                 for (int q = s; q < e; q++) {
                     int j = nl.cell_verts[q];
                     d3 d = pi - d3{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-                    if (norm(d) < dmax && j != i && !excluded(i, j)) f(j, d);
+                    double sq = sqnorm(d);
+                    if (sq <= dmax2 && sqrt(sq) < dmax && j != i && !excluded(i, j)) f(j, d);
                 }
             }
         }
@@ -286,7 +301,8 @@ __device__ __forceinline__ void nlist_visit(const NListView& nl, const double* _
         for (int q = s; q < e; q++) {
             int j = nl.adj[q];
             d3 d = pi - d3{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-            if (norm(d) < dmax) f(j, d);
+            double sq = sqnorm(d);
+            if (sq <= dmax2 && sqrt(sq) < dmax) f(j, d);
         }
     }
 }
