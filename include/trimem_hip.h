@@ -156,6 +156,12 @@ int tm_nlist_point_distances(tm_nlist* nl, tm_mesh* mesh, int pid, double dmax, 
 /* flip(mesh, estore, flip_ratio) with the random stream drawn from seed
  * (the reference draws from a clock-seeded mt19937, flips.cpp:20) */
 int tm_flip_serial(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int* accepted);
+/* pflip(mesh, estore, flip_ratio) (flip_parallel_batches, flips.cpp:82-166):
+ * rounds of `batch` proposals (<= 0: automatic); in each round proposals
+ * claim their flip patch (flip_utils.cpp:43-68), the winners (disjoint
+ * patches) flip in parallel and a sequential FP64 Metropolis scan in
+ * proposal order accepts or rejects them.  flip_ratio > 1 -> TM_E_RANGE. */
+int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int batch, int* accepted);
 /* flip_serial with caller-supplied proposals (edge ids) and uniforms; a
  * uniform is consumed only by proposals that pass is_flip_ok */
 int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const int* proposals,

@@ -294,3 +294,81 @@ def test_flip_smoke_reference_bound(core):
                  bond_lc0=1.15 * l, bond_lc1=0.85 * l, bond_a0=a)
     ge = core.EnergyManager(gm, to_core_params(core, p))
     assert core.flip(gm, ge, 0.5, seed=1) > 100
+
+
+# ------------------------------------------------------ parallel flips -----
+def check_manifold(f, nv, ne):
+    assert nv - ne + len(f) == 2
+    dirs = set()
+    for a, b, c in f:
+        assert len({a, b, c}) == 3
+        for x, y in ((a, b), (b, c), (c, a)):
+            assert (x, y) not in dirs
+            dirs.add((x, y))
+    for (x, y) in dirs:
+        assert (y, x) in dirs
+
+
+def test_pflip_deterministic_and_manifold(core):
+    v, f, p = flip_case(3, 21, True)
+    runs = []
+    for _ in range(2):
+        gm = core.TriMesh(v, f)
+        ge = core.EnergyManager(gm, to_core_params(core, p))
+        accs = [core.pflip(gm, ge, 0.3, seed=1234 + k) for k in range(5)]
+        runs.append((accs, gm.fv_indices().copy()))
+        check_manifold(gm.fv_indices(), gm.n_vertices(), gm.n_edges())
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert sum(runs[0][0]) > 0
+
+
+def test_pflip_topology_consistent_with_oracle(core):
+    """After parallel sweeps the device topology, ring table and host mirror
+    agree: properties, energy and gradient match an oracle mesh rebuilt from
+    fv_indices (area/volume/curvature penalties off, so the reference values
+    of the two managers do not enter)."""
+    v, f, p = flip_case(3, 22, True)
+    p.kappa_a = p.kappa_v = p.kappa_c = 0.0
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    for k in range(4):
+        core.pflip(gm, ge, 0.5, seed=77 + k)
+    fv = gm.fv_indices()
+    assert not np.array_equal(fv, f)
+    om = O.Mesh(v, fv)
+    oe = O.EnergyManager(om, p)
+    assert rel(props_vec(ge.properties(gm)), oe.properties(om)) < 1e-10
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) < 1e-10
+
+
+def test_flip_ratio_errors(core):
+    """flips.cpp:24-25 only constructs the error for flip; pflip raises
+    (flips.cpp:84-85, std::range_error -> ValueError)."""
+    v, f, p = flip_case(2, 23, False)
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    with pytest.raises(ValueError):
+        core.pflip(gm, ge, 1.5, seed=1)
+    core.flip(gm, ge, 1.2, seed=1)
+    check_manifold(gm.fv_indices(), gm.n_vertices(), gm.n_edges())
+
+
+def test_pflip_statistics_vs_serial(core):
+    """Both sweeps sample the same flip Markov chain; their mean acceptance
+    per attempted flip and mean energy over 40 sweeps agree statistically."""
+    v, f, p = flip_case(3, 24, False)
+    stats = {}
+    for kind in ("serial", "parallel"):
+        gm = core.TriMesh(v, f)
+        ge = core.EnergyManager(gm, to_core_params(core, p))
+        accs, ens = [], []
+        for k in range(40):
+            fn = core.flip if kind == "serial" else core.pflip
+            accs.append(fn(gm, ge, 0.2, seed=1000 + k))
+            ens.append(ge.energy(gm))
+        stats[kind] = (np.mean(accs[10:]), np.mean(ens[10:]))
+    (a_s, e_s), (a_p, e_p) = stats["serial"], stats["parallel"]
+    assert abs(a_p - a_s) < 0.35 * a_s, stats
+    assert abs(e_p - e_s) < 0.05 * abs(e_s), stats

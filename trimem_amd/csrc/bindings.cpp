@@ -515,6 +515,17 @@ PYBIND11_MODULE(core, m)
         py::arg("mesh"), py::arg("estore"), py::arg("flip_ratio"), py::kw_only(), py::arg("seed") = py::none());
 
     m.def(
+        "pflip",
+        [](TriMesh& t, EnergyManager& e, double ratio, py::object seed, int batch) {
+            uint64_t s = seed.is_none() ? fresh_seed() : seed.cast<uint64_t>();
+            int acc = 0;
+            check(tm_flip_parallel(e.h, t.h, ratio, s, batch, &acc));
+            return acc;
+        },
+        py::arg("mesh"), py::arg("estore"), py::arg("flip_ratio"), py::kw_only(), py::arg("seed") = py::none(),
+        py::arg("batch") = 0);
+
+    m.def(
         "flip_injected",
         [](TriMesh& t, EnergyManager& e, py::array_t<int, py::array::c_style | py::array::forcecast> proposals,
            py::array_t<double, py::array::c_style | py::array::forcecast> uniforms) {

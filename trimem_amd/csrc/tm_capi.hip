@@ -496,8 +496,9 @@ struct tm_energy {
     DevBuf<int> pair_cnt;
     DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
-    DevBuf<int> flip_result, proposals;
-    DevBuf<double> uniforms;
+    DevBuf<int> flip_result, proposals, pf_status;
+    DevBuf<double> uniforms, pf_dprops, pf_state;
+    DevBuf<unsigned long long> pf_owner;
     ~tm_energy() { delete nlist; }
 };
 
@@ -1040,6 +1041,46 @@ static int flip_impl(tm_energy* em, tm_mesh* mesh, int nflips, const int* propos
     if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
     if (accepted) *accepted = res[0];
     if (n_used) *n_used = res[1];
+    API_END
+}
+
+int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int batch, int* accepted)
+{
+    API_BEGIN
+    // flips.cpp:84-85
+    if (flip_ratio > 1.0) throw TmError(TM_E_RANGE, "flip_ratio must be <= 1.0");
+    const int ne = mesh->nh / 2;
+    const int nflips = (int)(ne * flip_ratio);
+    // a patch conflicts with proposals on ~100 nearby edges: B = E / 1024
+    // keeps the expected fraction of skipped proposals near 10%
+    if (batch <= 0) batch = std::max(1, ne / 1024);
+    upload_points(*mesh);
+    run_pass1(*em, *mesh, 0, nullptr);
+    if (!em->pf_owner.p || em->pf_owner.n < (size_t)ne) {
+        em->pf_owner.alloc(ne);
+        HIPCHECK(hipMemsetAsync(em->pf_owner.p, 0, sizeof(unsigned long long) * ne, g_rt.stream));
+    } else {
+        HIPCHECK(hipMemsetAsync(em->pf_owner.p, 0, sizeof(unsigned long long) * ne, g_rt.stream));
+    }
+    em->pf_status.alloc(batch);
+    em->pf_dprops.alloc((size_t)6 * batch);
+    em->pf_state.alloc(8);
+    if (nflips > 0)
+        launch_flip_parallel(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips, batch, seed,
+                             em->globals.p, em->pf_owner.p, em->pf_status.p, em->pf_dprops.p, em->pf_state.p,
+                             g_rt.stream);
+    check_launch();
+    HIPCHECK(hipMemsetAsync(mesh->flag.p, 0, sizeof(int), g_rt.stream));
+    launch_build_ring(mesh->view(), mesh->ring.p, mesh->meta.p, mesh->flag.p, g_rt.stream);
+    check_launch();
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (nflips > 0) d2h(st, em->pf_state.p, sizeof(st));
+    int ovf = 0;
+    d2h(&ovf, mesh->flag.p, sizeof(int));
+    sync();
+    mesh->host_stale = true;
+    if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
+    if (accepted) *accepted = (int)st[7];
     API_END
 }
 

@@ -16,6 +16,8 @@
 #include "tm_internal.h"
 #include "tm_om.h"
 
+#include <algorithm>
+
 namespace tmh {
 
 // ------------------------------------------------------- ring rebuild -----
@@ -258,6 +260,175 @@ void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EPara
 {
     hipLaunchKernelGGL(k_flip_serial, dim3(1), dim3(64), 0, s, fm, nl, p, nflips, proposals, uniforms, n_uniforms,
                        seed, injected, g0, result);
+}
+
+// ============================================ conflict-free parallel sweep ==
+// pflip (flip_parallel_batches, flips.cpp:82-166) lets OpenMP threads lock
+// the edge patch of their proposal (flip_patch, flip_utils.cpp:43-68), skip
+// proposals whose patch is taken, flip and evaluate in parallel and apply the
+// Metropolis test one flip at a time.  Here a round of B proposals is one
+// batch:
+//   claim   every proposal writes its priority key into all edges of its
+//           patch with a 64-bit atomicMax (later rounds have larger keys, so
+//           the owner array never needs clearing);
+//   eval    a proposal whose key survived on its whole patch is a winner;
+//           winners have pairwise disjoint patches.  Winners that pass
+//           is_flip_ok evaluate the patch properties before the flip;
+//   apply   winners flip;
+//   eval2   winners evaluate the patch after the flip: delta = new - old;
+//   scan    one lane walks the winners in proposal order and applies the
+//           FP64 Metropolis test to the running global properties;
+//   revert  rejected winners flip back.
+// Proposals that lose the claim are skipped, like the reference's failed
+// locks.  Every phase is its own kernel, so no phase reads topology that
+// another thread is rewriting; the result depends only on the seed.
+
+// Edges of the flip patch of edge e (flip_utils.cpp:43-68): for the two
+// endpoints' and the two opposite vertices' outgoing halfedges h, the edges
+// of h and of next(h).
+template <class F>
+__device__ __forceinline__ void for_patch_edges(const Topo& t, int e, F&& f)
+{
+    for (int i = 0; i < 2; i++) {
+        int heh = 2 * e + i;
+        for (int w = 0; w < 2; w++) {
+            int vh = t.to(heh);
+            int h0 = t.v_he[vh];
+            int h = h0;
+            int guard = 0;
+            do {
+                f(h >> 1);
+                f(t.next(h) >> 1);
+                h = t.cw(h);
+            } while (h != h0 && ++guard < 4 * kMaxValence);
+            heh = t.next(heh);
+        }
+    }
+}
+
+__device__ __forceinline__ int pflip_edge(uint64_t seed, int round, int B, int i, int ne)
+{
+    return rng_edge(seed ^ 0x5bd1e9955bd1e995ull, (uint64_t)round * (uint64_t)B + (uint64_t)i, ne);
+}
+
+__global__ void k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
+                              unsigned long long* __restrict__ owner)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprop) return;
+    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
+    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
+    if (t.edge_boundary(e)) return;
+    unsigned long long key = ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
+    for_patch_edges(t, e, [&](int pe) { atomicMax(&owner[pe], key); });
+}
+
+// status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected
+__global__ void k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed, int round, int B, int nprop,
+                             const unsigned long long* __restrict__ owner, int* __restrict__ status,
+                             double* __restrict__ dprops)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprop) return;
+    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
+    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
+    status[i] = 0;
+    if (t.edge_boundary(e)) return;
+    unsigned long long key = ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
+    bool win = true;
+    for_patch_edges(t, e, [&](int pe) { win = win && owner[pe] == key; });
+    if (!win || !flip_ok(t, e)) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < 4; s++) {
+        int h = 2 * e + (s >> 1);
+        int ve = (s & 1) ? t.to(t.next(h)) : t.to(h);
+        VP r = om_vertex_properties(t, nl, p, ve);
+        for (int k = 0; k < 6; k++) acc[k] += r.v[k];
+    }
+    for (int k = 0; k < 6; k++) dprops[6 * i + k] = -acc[k];
+    status[i] = 1;
+}
+
+__global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
+                              const int* __restrict__ status, int want)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprop || status[i] != want) return;
+    do_flip(fm, pflip_edge(seed, round, B, i, fm.nh / 2));
+}
+
+__global__ void k_pflip_eval2(FlipMeshView fm, NListView nl, EParams p, uint64_t seed, int round, int B, int nprop,
+                              const int* __restrict__ status, double* __restrict__ dprops)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprop || status[i] != 1) return;
+    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
+    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < 4; s++) {
+        int h = 2 * e + (s >> 1);
+        int ve = (s & 1) ? t.to(t.next(h)) : t.to(h);
+        VP r = om_vertex_properties(t, nl, p, ve);
+        for (int k = 0; k < 6; k++) acc[k] += r.v[k];
+    }
+    for (int k = 0; k < 6; k++) dprops[6 * i + k] += acc[k];
+}
+
+// state: props[6], e0, accepted count (as double)
+__global__ void k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop, int* __restrict__ status,
+                             const double* __restrict__ dprops, double* __restrict__ state)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double props[6];
+    for (int k = 0; k < 6; k++) props[k] = state[k];
+    double e0 = state[6];
+    double acc = state[7];
+    for (int i = 0; i < nprop; i++) {
+        if (status[i] != 1) continue;
+        for (int k = 0; k < 6; k++) props[k] += dprops[6 * i + k];
+        double en = energy_of(props, p);
+        double de = en - e0;
+        double alpha = de < 0.0 ? 1.0 : exp(-de);
+        double u = rng_uniform(seed ^ 0x5bd1e9955bd1e995ull, (uint64_t)round * (uint64_t)B + (uint64_t)i);
+        if (u <= alpha) {
+            e0 = en;
+            acc += 1.0;
+            status[i] = 2;
+        } else {
+            for (int k = 0; k < 6; k++) props[k] -= dprops[6 * i + k];
+            status[i] = 3;
+        }
+    }
+    for (int k = 0; k < 6; k++) state[k] = props[k];
+    state[6] = e0;
+    state[7] = acc;
+}
+
+__global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* __restrict__ state)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double props[6];
+    for (int k = 0; k < 6; k++) props[k] = state[k] = g0->props[k];
+    state[6] = energy_of(props, p);
+    state[7] = 0.0;
+}
+
+void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
+                          uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
+                          double* state, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pflip_init, dim3(1), dim3(64), 0, s, g0, p, state);
+    const int rounds = (nflips + batch - 1) / batch;
+    for (int r = 0; r < rounds; r++) {
+        int nprop = std::min(batch, nflips - r * batch);
+        dim3 grid((nprop + 127) / 128), block(128);
+        hipLaunchKernelGGL(k_pflip_claim, grid, block, 0, s, fm, seed, r, batch, nprop, owner);
+        hipLaunchKernelGGL(k_pflip_eval, grid, block, 0, s, fm, nl, p, seed, r, batch, nprop, owner, status, dprops);
+        hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 1);
+        hipLaunchKernelGGL(k_pflip_eval2, grid, block, 0, s, fm, nl, p, seed, r, batch, nprop, status, dprops);
+        hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(64), 0, s, p, seed, r, batch, nprop, status, dprops, state);
+        hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 3);
+    }
 }
 
 }    // namespace tmh

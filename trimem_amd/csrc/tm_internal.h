@@ -70,4 +70,10 @@ void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EPara
                         const int* proposals, const double* uniforms, int n_uniforms, uint64_t seed, int injected,
                         const Globals* g0, int* result /* [accepted, used, error] */, hipStream_t s);
 
+// graph-coloured (claim-based) parallel sweep; state receives props[6], the
+// final energy and the number of accepted flips
+void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
+                          uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
+                          double* state, hipStream_t s);
+
 }    // namespace tmh
