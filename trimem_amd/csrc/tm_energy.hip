@@ -30,6 +30,8 @@
 #include "tm_internal.h"
 #include "tm_om.h"
 
+#include <algorithm>
+
 namespace tmh {
 
 // Scalar kernels of the two precision modes.  Geometry vectors (edge vectors,
@@ -254,16 +256,52 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
         }
     };
     if (nl.kind == 0) {
+        // bounds of the 9 cell rows: lane r loads row r (lane 0 also row 8),
+        // the group shares them; candidates of all rows form one index range
+        // that is strided over the lanes, so the lanes stay balanced
         int c[3];
         cell_coords(nl, pi, c);
         const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
-        if (i0 > i1) return;
-        for (int r = lane; r < 9; r += kGroup) {
+        auto row = [&](int r, int& s0, int& len) {
             int kk = c[2] + r / 3 - 1;
             int jj = c[1] + r % 3 - 1;
-            if (kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) continue;
+            s0 = 0;
+            len = 0;
+            if (i0 > i1 || kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) return;
             int base = jj * nl.strides[1] + kk * nl.strides[2];
-            scan(nl.cell_verts, nl.cell_start[base + i0], nl.cell_start[base + i1 + 1], 1);
+            s0 = nl.cell_start[base + i0];
+            len = nl.cell_start[base + i1 + 1] - s0;
+        };
+        int my_s, my_len, s8, len8;
+        row(lane, my_s, my_len);
+        row(8, s8, len8);
+        const int base_lane = threadIdx.x & ~(kGroup - 1);
+        int rs[9], rl[9];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            rs[r] = __shfl(my_s, base_lane + r, 64);
+            rl[r] = __shfl(my_len, base_lane + r, 64);
+        }
+        rs[8] = s8;
+        rl[8] = len8;
+        int total = 0;
+#pragma unroll
+        for (int r = 0; r < 9; r++) total += rl[r];
+        for (int t = lane; t < total; t += 2 * kGroup) {
+            int q0 = -1, q1 = -1;
+            int acc = 0;
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                if (t >= acc && t < acc + rl[r]) q0 = rs[r] + (t - acc);
+                if (t + kGroup >= acc && t + kGroup < acc + rl[r]) q1 = rs[r] + (t + kGroup - acc);
+                acc += rl[r];
+            }
+            int j0 = nl.cell_verts[q0];
+            int j1 = q1 >= 0 ? nl.cell_verts[q1] : j0;
+            d3 p0 = load_p(m.x, j0);
+            d3 p1 = load_p(m.x, j1);
+            test(j0, p0);
+            if (q1 >= 0) test(j1, p1);
         }
     } else if (nl.kind == 1) {
         scan(nl.adj, nl.adj_start[v] + lane, nl.adj_start[v + 1], kGroup);
