@@ -39,6 +39,10 @@ def workload(level):
     return v, f, lbar, params
 
 
+PROFILE_JSON = os.path.join(ROOT, "profiles", "pmc_energy_L7_mixed.json")
+COPY_BYTES = 64 << 20
+
+
 class Hip:
     """hipEvent timing on the library's stream."""
 
@@ -48,6 +52,22 @@ class Hip:
         self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.h.hipFree.argtypes = [ctypes.c_void_p]
+        self.h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                          ctypes.c_void_p]
+        self.h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+
+    def copy_calibration(self, stream, nbytes=COPY_BYTES):
+        """One device-to-device copy of known size (the runtime's copy
+        kernel), so that HBM counters of the same run can be scaled."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert self.h.hipMalloc(ctypes.byref(a), nbytes) == 0
+        assert self.h.hipMalloc(ctypes.byref(b), nbytes) == 0
+        assert self.h.hipMemcpyAsync(b, a, nbytes, 3, stream) == 0    # hipMemcpyDeviceToDevice
+        assert self.h.hipStreamSynchronize(stream) == 0
+        self.h.hipFree(a)
+        self.h.hipFree(b)
 
     def event(self):
         e = ctypes.c_void_p()
@@ -84,6 +104,74 @@ def cpu_baseline(v, f, params, budget_s):
     return 1.0 / float(np.median(times)), len(times), O.num_threads()
 
 
+def _csv_rows(path):
+    import csv
+    return list(csv.DictReader(open(path)))
+
+
+def _find(d, suffix):
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith(suffix):
+                return os.path.join(root, f)
+    raise FileNotFoundError(f"{suffix} under {d}")
+
+
+def _short(name):
+    if "copyBuffer" in name:
+        return "copy"
+    for k in ("k_pass1", "k_pass2", "k_finalize", "k_generic_props", "k_generic_grad"):
+        if k in name:
+            return k + ("_fp64" if "<double>" in name else "")
+    return None
+
+
+def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command):
+    """Summaries of rocprofv3 runs of this script: average kernel durations
+    (--kernel-trace --stats) and HBM KiB per dispatch (--pmc FETCH_SIZE, then
+    WRITE_SIZE), scaled by what the counters read for the calibration copy
+    of COPY_BYTES."""
+    stats = {}
+    for r in _csv_rows(_find(trace_dir, "kernel_stats.csv")):
+        k = _short(r["Name"])
+        if k:
+            stats[k] = {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3}
+    pmc = {}
+    for d, cname in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
+        for r in _csv_rows(_find(d, "counter_collection.csv")):
+            k = _short(r.get("Kernel_Name", r.get("Kernel-Name", "")))
+            if k and r["Counter_Name"] == cname:
+                pmc.setdefault(k, {}).setdefault(cname, []).append(float(r["Counter_Value"]))
+    copy_kib = COPY_BYTES / 1024.0
+    f_fetch = float(np.mean(pmc["copy"]["FETCH_SIZE"])) / copy_kib
+    f_write = float(np.mean(pmc["copy"]["WRITE_SIZE"])) / copy_kib
+    kernels = {}
+    for k, c in pmc.items():
+        if k == "copy":
+            continue
+        fk = float(np.mean(c["FETCH_SIZE"]))
+        wk = float(np.mean(c["WRITE_SIZE"]))
+        kernels[k] = {"fetch_kib": fk, "write_kib": wk,
+                      "hbm_bytes_per_dispatch": (fk / f_fetch + wk / f_write) * 1024.0,
+                      "avg_us": stats.get(k, {}).get("avg_us")}
+    out = {"command": command, "copy_bytes": COPY_BYTES, "copy_factor_fetch": f_fetch,
+           "copy_factor_write": f_write, "kernels": kernels, "durations": stats}
+    json.dump(out, open(out_json, "w"), indent=1)
+    lines = ["# rocprofv3 summary: energy + gradient passes, icosphere L7, mixed precision", "",
+             "Commands (run from the repository root on one MI355X):", "", "```"] + command + ["```", "",
+             f"Copy calibration: {COPY_BYTES} B device-to-device copy reads FETCH_SIZE x {f_fetch:.4f} "
+             f"and WRITE_SIZE x {f_write:.4f} of its size; kernel traffic below is divided by these factors.", "",
+             "| kernel | calls | avg duration (us) | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes / dispatch |",
+             "|---|---|---|---|---|---|"]
+    for k in sorted(stats):
+        kk = kernels.get(k, {})
+        lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
+                     f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
+                     f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
+    open(out_md, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +181,20 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/ summaries from three rocprofv3 output folders and exit")
     args = ap.parse_args()
+    if args.summarize:
+        cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
+               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline",
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + args.summarize[1] +
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline",
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + args.summarize[2] +
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline",
+               "python3 bench.py --summarize " + " ".join(args.summarize)]
+        summarize(*args.summarize, PROFILE_JSON, os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"),
+                  cmd)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: every box here has a single MI355X; "
               "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
@@ -120,6 +221,7 @@ def main():
     stream = ctypes.c_void_p()
     capi.check(L.tm_stream(ctypes.byref(stream)))
     hip = Hip()
+    hip.copy_calibration(stream)
     both = capi.TM_EVAL_PROPS | capi.TM_EVAL_GRAD
 
     for _ in range(args.warmup):
@@ -165,6 +267,12 @@ def main():
     dom = ("k_pass2", b_pass2, t2) if t2 >= t1 else ("k_pass1", b_pass1, t1)
     achieved = dom[1] / dom[2] / 1e9
     eval_bytes = b_pass1 + b_pass2
+    traffic = None
+    if os.path.exists(PROFILE_JSON) and args.level == 7 and args.precision == "mixed":
+        prof = json.load(open(PROFILE_JSON))
+        k = prof["kernels"].get(dom[0])
+        if k is not None:
+            traffic = k["hbm_bytes_per_dispatch"]
 
     out = {
         "metric": "energy+grad evals/s",
@@ -187,7 +295,7 @@ def main():
         "hbm_gbps_per_eval": eval_bytes * evals_per_s / 1e9,
         "kernels_ms": {"k_pass1": float(np.mean(p1)), "k_pass2": float(np.mean(p2))},
         "roofline": {"kernel": dom[0], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes": dom[1]},
         "wall_s_timed": wall,
     }
