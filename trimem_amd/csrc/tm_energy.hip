@@ -147,19 +147,65 @@ __device__ void finalize_globals(const double* __restrict__ partials, int nblock
 // round trips, and the group sums its per-slot terms with lane shuffles.
 constexpr int kGroup = 8;
 
+// Sums over the 8 lanes of a group with DPP lane moves (no LDS traffic):
+// xor 1 and xor 2 within quads, then each lane i of the group pairs with lane
+// 7 - i (row_half_mirror), which joins the two quads.  Every lane ends with
+// the same bits (the last step adds the same two quad sums, a + b == b + a).
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(dpp_i<CTRL>(hi), dpp_i<CTRL>(lo));
+}
 __device__ __forceinline__ double group_sum(double x)
 {
-    x += __shfl_xor(x, 4, kGroup);
-    x += __shfl_xor(x, 2, kGroup);
-    x += __shfl_xor(x, 1, kGroup);
+    x += dpp_d<kDppXor1>(x);
+    x += dpp_d<kDppXor2>(x);
+    x += dpp_d<kDppHalfMirror>(x);
     return x;
 }
 __device__ __forceinline__ int group_sum(int x)
 {
-    x += __shfl_xor(x, 4, kGroup);
-    x += __shfl_xor(x, 2, kGroup);
-    x += __shfl_xor(x, 1, kGroup);
+    x += dpp_i<kDppXor1>(x);
+    x += dpp_i<kDppXor2>(x);
+    x += dpp_i<kDppHalfMirror>(x);
     return x;
+}
+
+// Block sum of six doubles held by the group leaders (lane % 8 == 0); the
+// result is in s_out[0..5].
+__device__ __forceinline__ void block_sum6_leaders(double v[6], double* s_red, double* s_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double a = v[k];
+        a += __shfl_xor(a, 8, 64);
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        v[k] = a;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) s_red[wave * 6 + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double a = 0;
+        for (int w = 0; w < nw; w++) a += s_red[w * 6 + threadIdx.x];
+        s_out[threadIdx.x] = a;
+    }
+    __syncthreads();
 }
 
 // Exclusion test of the cell list (cell_list.h:170-202) on the ring table:
@@ -377,6 +423,8 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
     rep = group_sum(rep);
     cnt = group_sum(cnt);
     double pr[6] = {0, 0, 0, 0, 0, 0};
+    __shared__ double s_q[256 / kGroup];
+    __shared__ int s_cnt[256 / kGroup];
     if (active && lane == 0) {
         if (ring) {
             double a = area / 3, c = curv / 2;
@@ -386,16 +434,25 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
             pr[3] = 2 * c * c / a;
             pr[4] = teth / (p.bond_type == 0 ? 2 : 3);
             pr[5] = rep;
-            pair_cnt[v] = cnt;
         } else {
             for (int k = 0; k < 6; k++) pr[k] = gprops[6 * v + k];
         }
-        q[v] = pr[2] / pr[0];
+        s_q[threadIdx.x / kGroup] = pr[2] / pr[0];
+        s_cnt[threadIdx.x / kGroup] = ring ? cnt : 0;
         if (vprops) {
             for (int k = 0; k < 6; k++) vprops[6 * v + k] = pr[k];
         }
     }
-    block_sum6(pr, s_red, s_out);
+    block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q / s_cnt
+    {
+        // the block's 32 consecutive vertices, written contiguously
+        const int v0 = (logical_block() * blockDim.x) / kGroup;
+        const int i = threadIdx.x;
+        if (i < 256 / kGroup && v0 + i < m.nv) {
+            q[v0 + i] = s_q[i];
+            pair_cnt[v0 + i] = s_cnt[i];
+        }
+    }
     if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = s_out[threadIdx.x];
 }
 
@@ -522,10 +579,23 @@ __global__ void __launch_bounds__(256) k_pass2(MeshView m, NListView nl, EParams
     acc.x = group_sum(acc.x);
     acc.y = group_sum(acc.y);
     acc.z = group_sum(acc.z);
-    if (ring && lane == 0) {
-        grad[3 * v] = acc.x;
-        grad[3 * v + 1] = acc.y;
-        grad[3 * v + 2] = acc.z;
+    // stage the block's 32 gradient rows and store them as contiguous doubles;
+    // rows of listed (boundary) vertices are left to k_generic_grad
+    __shared__ double s_g[3 * 256 / kGroup];
+    __shared__ int s_ring[256 / kGroup];
+    if (lane == 0) {
+        const int r = threadIdx.x / kGroup;
+        s_g[3 * r] = acc.x;
+        s_g[3 * r + 1] = acc.y;
+        s_g[3 * r + 2] = acc.z;
+        s_ring[r] = ring;
+    }
+    __syncthreads();
+    const int v0 = (logical_block() * blockDim.x) / kGroup;
+    const int i = threadIdx.x;
+    if (i < 3 * 256 / kGroup) {
+        const int r = i / 3;
+        if (v0 + r < m.nv && s_ring[r]) grad[3 * v0 + i] = s_g[i];
     }
 }
 
