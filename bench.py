@@ -136,19 +136,23 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command):
         k = _short(r["Name"])
         if k:
             stats[k] = {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3}
-    pmc = {}
+    pmc, calib = {}, {}
     for d, cname in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
         for r in _csv_rows(_find(d, "counter_collection.csv")):
             k = _short(r.get("Kernel_Name", r.get("Kernel-Name", "")))
-            if k and r["Counter_Name"] == cname:
+            if not k or r["Counter_Name"] != cname:
+                continue
+            if k == "copy":
+                # the runtime issues small copies of its own; calibrate on
+                # the COPY_BYTES dispatch, the one with the largest grid
+                calib.setdefault(cname, []).append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+            else:
                 pmc.setdefault(k, {}).setdefault(cname, []).append(float(r["Counter_Value"]))
     copy_kib = COPY_BYTES / 1024.0
-    f_fetch = float(np.mean(pmc["copy"]["FETCH_SIZE"])) / copy_kib
-    f_write = float(np.mean(pmc["copy"]["WRITE_SIZE"])) / copy_kib
+    f_fetch = max(calib["FETCH_SIZE"])[1] / copy_kib
+    f_write = max(calib["WRITE_SIZE"])[1] / copy_kib
     kernels = {}
     for k, c in pmc.items():
-        if k == "copy":
-            continue
         fk = float(np.mean(c["FETCH_SIZE"]))
         wk = float(np.mean(c["WRITE_SIZE"]))
         kernels[k] = {"fetch_kib": fk, "write_kib": wk,
@@ -197,7 +201,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: every box here has a single MI355X; "
-              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+              "multi-GPU sharding is a design note, not a launch mode", file=sys.stderr)
         sys.exit(2)
 
     from trimem_amd import capi
