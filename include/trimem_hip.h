@@ -87,8 +87,15 @@ int tm_mesh_next_halfedge(tm_mesh* mesh, int he, int* out);
 int tm_mesh_topology(tm_mesh* mesh, int* he_to, int* he_next, int* he_face, int* v_he, int* f_he);
 /* device-resident positions: double[n*3] in HBM */
 double* tm_mesh_device_points(tm_mesh* mesh);
+/* host -> device (the host array wins) and device -> host copies of the
+ * positions */
 int tm_mesh_upload_points(tm_mesh* mesh);
 int tm_mesh_download_points(tm_mesh* mesh);
+/* After a device-resident step (tm_hmc_step) the device positions are ahead
+ * of the host array; this copies them back if so (no-op otherwise).  Every
+ * host-facing compute call does the same before it reads positions, instead
+ * of uploading the stale host array. */
+int tm_mesh_sync_host(tm_mesh* mesh);
 
 /* ---- geometry on one halfedge (mesh_util.h:15-185, bindings.cpp:244-407) ----
  * out[47] = [edge_length, edge_length_grad(2x3), face_area, face_area_grad(3x3),
@@ -166,6 +173,45 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
  * uniform is consumed only by proposals that pass is_flip_ok */
 int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const int* proposals,
                             const double* uniforms, int n_uniforms, int* accepted, int* n_used);
+
+/* ---- device-resident HMC step (trimem/mc/hmc.py:14-26, 129-158) ----
+ * One step of trimem's HMC sampler with positions, momenta and forces in HBM:
+ * momenta p ~ N(0, mass*T), num_integration_steps velocity-Verlet steps
+ * (hmc.py:14-26, forces from the energy passes), FP64 Metropolis test on
+ * H = E + 0.5 p.p / mass (hmc.py:129-131, 146-155), positions restored on
+ * rejection.  Enqueued on the library stream; no host synchronisation. */
+typedef struct tm_hmc tm_hmc;
+enum { TM_HMC_ENERGY = 0,      /* E = the energy manager's energy */
+       TM_HMC_HARMONIC = 1 };  /* E = 0.5 x.x (tests/test_sampling.py:40-44) */
+typedef struct tm_hmc_params {
+    double mass;               /* "mass" (hmc.py:38) */
+    double time_step;          /* "time_step" (hmc.py:39) */
+    double time_step_sq;       /* time_step squared as the caller forms it
+                                  (hmc.py:21 uses dt**2); 0 -> dt*dt */
+    int num_integration_steps; /* L >= 0 (hmc.py:40) */
+    int reserved;
+} tm_hmc_params;
+typedef struct tm_hmc_outcome {
+    double dh;                 /* (H(xn,pn) - H(x,p)) / T of the last step */
+    double energy0, energy1;   /* E(x), E(xn) */
+    double kinetic0, kinetic1; /* 0.5 p.p / mass at start and end */
+    double uniform;            /* the uniform compared with min(1, exp(-dh)) */
+    long long accepted;        /* last step: 1 accepted, 0 rejected */
+    long long n_accepted;      /* accepted steps since creation */
+    long long n_steps;
+} tm_hmc_outcome;
+/* em may be NULL for TM_HMC_HARMONIC; mesh and em must outlive the handle */
+int tm_hmc_create(tm_mesh* mesh, tm_energy* em, int model, tm_hmc** out);
+int tm_hmc_destroy(tm_hmc* hmc);
+/* momenta: double[n*3] in user vertex order, the caller's
+ * normal(size)*sqrt(mass*T) draw (host RNG, hmc.py:140), or NULL for a
+ * counter-based device draw keyed by (seed, step).  uniform: the caller's
+ * U[0,1) draw (hmc.py:151), or < 0 for a device draw keyed by (seed, step).
+ * Host momenta are copied before the call returns. */
+int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* params, double temperature, const double* momenta,
+                double uniform, uint64_t seed, uint64_t step);
+/* result of the last step and running counts (synchronises) */
+int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
 
 #ifdef __cplusplus
 }

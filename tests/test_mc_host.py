@@ -1,0 +1,169 @@
+"""Host side of the Monte Carlo drivers (trimem_amd.mc) on the CPU.
+
+* the reference sampler test (tests/test_sampling.py:32-90): seeded HMC and
+  MeshHMC on the icosahedron with E = x.x / 2 reproduce the committed
+  trajectory end point ``ref_hmc`` to 1e-14;
+* the integrator against the closed-form velocity-Verlet map of the harmonic
+  oscillator (a kick with the old force, a missing half, or a wrong dt power
+  fails it), and its energy error order;
+* the cooling schedule's bounds, the flip-type errors and the shared step
+  counter of the combined chain.
+"""
+import json
+import os
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from trimem_amd import meshes as M
+from trimem_amd.mc import hmc as H
+from trimem_amd.mc.mesh import Mesh
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+REF_HMC = np.array(GOLD["ref_hmc"]["x"])
+
+
+def _quadratic():
+    return (lambda x: 0.5 * x.ravel().dot(x.ravel())), (lambda x: x)
+
+
+def test_hmc_reference_trajectory():
+    np.random.seed(42)
+    p, c = M.icosahedron()
+    mesh = Mesh(p, c)
+    energy, gradient = _quadratic()
+    hmc = H.HMC(mesh.x, energy, gradient, counter=H.get_step_counters(),
+                options={"info_step": 10, "time_step": 1.0e-1})
+    hmc.run(10)
+    assert np.linalg.norm(hmc.x - REF_HMC) < GOLD["ref_hmc"]["tolerance"]
+
+
+def test_mesh_hmc_reference_trajectory():
+    np.random.seed(42)
+    p, c = M.icosahedron()
+    mesh = Mesh(p, c)
+
+    def energy(x):
+        mesh.x = x
+        return 0.5 * x.ravel().dot(x.ravel())
+
+    def gradient(x):
+        mesh.x = x
+        return x
+
+    hmc = H.MeshHMC(mesh, energy, gradient, counter=H.get_step_counters(),
+                    options={"info_step": 10, "time_step": 1.0e-1})
+    hmc.run(10)
+    assert np.linalg.norm(hmc.x - REF_HMC) < GOLD["ref_hmc"]["tolerance"]
+    assert np.array_equal(mesh.x, hmc.x)
+
+
+@pytest.mark.parametrize("dt,n", [(0.1, 1), (0.1, 7), (0.37, 13)])
+def test_velocity_verlet_harmonic_map(dt, n):
+    """For F = -x, m = 1 one Verlet step is the linear map
+    [[1 - h^2/2, h], [-h (1 - h^2/4), 1 - h^2/2]] on (x, p)."""
+    rng = np.random.default_rng(3)
+    x0, p0 = rng.normal(size=(5, 3)), rng.normal(size=(5, 3))
+    x, p = H._vv_integration(x0, p0, lambda y: -y, 1.0, dt, n)
+    A = np.array([[1 - dt**2 / 2, dt], [-dt * (1 - dt**2 / 4), 1 - dt**2 / 2]])
+    An = np.linalg.matrix_power(A, n)
+    assert np.allclose(x, An[0, 0] * x0 + An[0, 1] * p0, rtol=0, atol=1e-12)
+    assert np.allclose(p, An[1, 0] * x0 + An[1, 1] * p0, rtol=0, atol=1e-12)
+    assert abs(np.linalg.det(A) - 1.0) < 1e-15    # symplectic
+
+
+def test_velocity_verlet_mass_and_order():
+    """Mass m scales time as sqrt(m); the energy error after a fixed time
+    falls as dt^2."""
+    x0 = np.array([[1.0, 0.0, 0.0]])
+    p0 = np.array([[0.0, 0.5, 0.0]])
+    force = lambda y: -y
+    errs = []
+    for dt in (0.02, 0.01):
+        x, p = H._vv_integration(x0, p0, force, 2.0, dt, int(round(1.0 / dt)))
+        errs.append(abs((0.5 * (x * x).sum() + 0.25 * (p * p).sum()) - (0.5 + 0.25 * 0.25)))
+    assert 3.5 < errs[0] / errs[1] < 4.5
+
+
+def test_cooling_schedule():
+    energy, gradient = _quadratic()
+    c = H.get_step_counters()
+    hmc = H.HMC(np.zeros((2, 3)), energy, gradient, counter=c,
+                options={"cooling_factor": 0.5, "initial_temperature": 2.0, "minimal_temperature": 0.1,
+                         "cooling_start_step": 3, "num_integration_steps": 1})
+    temps = []
+    for _ in range(12):
+        hmc.step()
+        temps.append(hmc.T)
+    assert temps[:4] == [2.0] * 4    # T held at Tinit until the start step
+    assert all(a >= b for a, b in zip(temps, temps[1:]))
+    assert temps[-1] == 0.1          # clamped at Tmin
+    assert c["move"] == 12
+
+
+def test_mesh_flips_types():
+    p, c = M.icosahedron()
+    mesh = Mesh(p, c)
+    with pytest.raises(ValueError, match="Wrong flip-type"):
+        H.MeshFlips(mesh, None, counter=H.get_step_counters(), options={"flip_type": "diagonal"})
+    fl = H.MeshFlips(mesh, None, counter=H.get_step_counters(), options={"flip_type": "none"})
+    fl.run(3)
+    assert fl.counter["flip"] == 3 and fl.acc == 0
+    fl = H.MeshFlips(mesh, None, counter=H.get_step_counters(), options={"flip_ratio": 0.0})
+    fl.step()
+    assert fl.acc == 0
+
+
+def test_mesh_monte_carlo_shares_counter():
+    np.random.seed(7)
+    p, c = M.icosahedron()
+    mesh = Mesh(p, c)
+    energy, gradient = _quadratic()
+    hmc = H.MeshHMC(mesh, energy, gradient, counter=Counter(), options={"num_integration_steps": 2})
+    flips = H.MeshFlips(mesh, None, counter=Counter(), options={"flip_type": "none"})
+    seen = []
+    mc = H.MeshMonteCarlo(hmc, flips, counter=H.get_step_counters(), callback=lambda x, s: seen.append(dict(s)))
+    mc.run(20)
+    assert hmc.counter is mc.counter and flips.counter is mc.counter
+    assert mc.counter["move"] + mc.counter["flip"] == 20
+    assert 0 < mc.counter["move"] < 20
+    assert [sum(s.values()) for s in seen] == list(range(1, 21))
+
+
+def test_mesh_hmc_state_aliases_mesh_view():
+    """MeshHMC's initial state is the mesh's points view (hmc.py:221); with
+    evaluators that write x into the mesh (evaluators.py:105) the trajectory
+    overwrites it, so H(x, p) of the first step is evaluated at the
+    trajectory's end point.  The mirror keeps this behaviour."""
+    np.random.seed(0)
+    p, c = M.icosahedron()
+    mesh = Mesh(p, c)
+    energies = []
+
+    def energy(x):
+        mesh.x = x
+        energies.append(0.5 * x.ravel().dot(x.ravel()))
+        return energies[-1]
+
+    def gradient(x):
+        mesh.x = x
+        return x
+
+    hmc = H.MeshHMC(mesh, energy, gradient, counter=H.get_step_counters(),
+                    options={"time_step": 0.3, "num_integration_steps": 5})
+    assert np.shares_memory(hmc.x, mesh.x)
+    hmc.step()
+    assert len(energies) == 2 and energies[0] == energies[1]
+    assert energies[1] != 0.5 * (p * p).sum()
+
+
+def test_hmc_does_not_invoke_callback():
+    """hmc.py:98 stores a lambda that returns the callback without calling
+    it; HMC.run therefore never calls it."""
+    calls = []
+    energy, gradient = _quadratic()
+    hmc = H.HMC(np.ones((2, 3)), energy, gradient, callback=lambda x, s: calls.append(1),
+                counter=H.get_step_counters(), options={"num_integration_steps": 1})
+    hmc.run(3)
+    assert calls == []

@@ -80,6 +80,7 @@ py::array_t<double> points_view(py::object self)
 {
     TriMesh& m = self.cast<TriMesh&>();
     size_t n = (size_t)m.nv();
+    check(tm_mesh_sync_host(m.h));    // newer device positions, if any
     return py::array_t<double>({n, (size_t)3}, {3 * sizeof(double), sizeof(double)}, tm_mesh_points(m.h), self);
 }
 
@@ -335,6 +336,57 @@ struct EnergyManager {
     }
 };
 
+// One HMC step of trimem.mc.hmc (hmc.py:133-158) run on the device; the
+// Python samplers in trimem_amd.mc drive it.
+struct DeviceHMC {
+    tm_hmc* h = nullptr;
+    int nv = 0;
+    DeviceHMC(TriMesh& m, EnergyManager* e, const std::string& model) : nv(m.nv())
+    {
+        int md;
+        if (model == "energy") md = TM_HMC_ENERGY;
+        else if (model == "harmonic") md = TM_HMC_HARMONIC;
+        else throw std::runtime_error("Unknown HMC energy model: " + model);
+        check(tm_hmc_create(m.h, e ? e->h : nullptr, md, &h));
+    }
+    DeviceHMC(const DeviceHMC&) = delete;
+    ~DeviceHMC() { tm_hmc_destroy(h); }
+
+    void step(double mass, double time_step, int num_integration_steps, double temperature, py::object momenta,
+              double uniform, uint64_t seed, uint64_t step, double time_step_sq)
+    {
+        tm_hmc_params p{};
+        p.mass = mass;
+        p.time_step = time_step;
+        p.time_step_sq = time_step_sq;
+        p.num_integration_steps = num_integration_steps;
+        if (momenta.is_none()) {
+            check(tm_hmc_step(h, &p, temperature, nullptr, uniform, seed, step));
+            return;
+        }
+        auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(momenta);
+        if (!a || a.ndim() != 2 || a.shape(0) != nv || a.shape(1) != 3)
+            throw std::runtime_error("momenta must have shape (n_vertices, 3)");
+        check(tm_hmc_step(h, &p, temperature, a.data(), uniform, seed, step));
+    }
+    py::dict result()
+    {
+        tm_hmc_outcome o;
+        check(tm_hmc_result(h, &o));
+        py::dict d;
+        d["accepted"] = (bool)o.accepted;
+        d["dh"] = o.dh;
+        d["energy0"] = o.energy0;
+        d["energy1"] = o.energy1;
+        d["kinetic0"] = o.kinetic0;
+        d["kinetic1"] = o.kinetic1;
+        d["uniform"] = o.uniform;
+        d["n_accepted"] = o.n_accepted;
+        d["n_steps"] = o.n_steps;
+        return d;
+    }
+};
+
 std::atomic<uint64_t> g_seed_counter{0};
 uint64_t fresh_seed()
 {
@@ -549,6 +601,17 @@ PYBIND11_MODULE(core, m)
     py::class_<NeighbourList>(m, "NeighbourList")
         .def("distance_matrix", &NeighbourList::distance_matrix, py::arg("mesh"), py::arg("rdist"))
         .def("point_distances", &NeighbourList::point_distances, py::arg("mesh"), py::arg("pid"), py::arg("rdist"));
+
+    py::class_<DeviceHMC>(m, "DeviceHMC")
+        .def(py::init([](TriMesh& t, EnergyManager* e, const std::string& model) {
+                 return std::make_unique<DeviceHMC>(t, e, model);
+             }),
+             py::arg("mesh"), py::arg("estore") = nullptr, py::arg("model") = "energy", py::keep_alive<1, 2>(),
+             py::keep_alive<1, 3>())
+        .def("step", &DeviceHMC::step, py::kw_only(), py::arg("mass"), py::arg("time_step"), py::arg("num_integration_steps"),
+            py::arg("temperature"), py::arg("momenta") = py::none(), py::arg("uniform") = -1.0,
+            py::arg("seed") = 0, py::arg("step") = 0, py::arg("time_step_sq") = 0.0)
+        .def("result", &DeviceHMC::result);
 
     m.def(
         "make_nlist",

@@ -112,6 +112,7 @@ struct tm_mesh {
     // host mirror of the topology
     std::vector<int> he_to, he_next, he_prev, he_face, v_he, f_he;
     bool host_stale = false;     // device topology changed (flips)
+    bool device_ahead = false;   // device positions moved by a device-resident step
     bool has_boundary = false;
     // device state
     bool dev = false;
@@ -328,13 +329,27 @@ void ensure_device(tm_mesh& M)
     M.dev = true;
 }
 
-void upload_points(tm_mesh& M)
+void push_points(tm_mesh& M)
 {
     ensure_device(M);
+    M.device_ahead = false;
     if (M.nv == 0) return;
     h2d(M.stage.p, M.points, sizeof(double) * 3 * M.nv);
     launch_gather_rows(M.stage.p, M.d_perm.p, M.x.p, M.nv, 3, g_rt.stream);
     check_launch();
+}
+
+void pull_points(tm_mesh& M);
+
+// Host-facing calls read positions from the host array (the reference's
+// points() view, mesh_py.cpp:96-105).  When a device-resident step has moved
+// the device copy since, the device is the newer state: copy it back to the
+// host array instead of overwriting it.
+void upload_points(tm_mesh& M)
+{
+    ensure_device(M);
+    if (M.device_ahead) pull_points(M);
+    else push_points(M);
 }
 
 // rows of an internal-order device array -> user order in the stage buffer
@@ -345,6 +360,13 @@ void download_rows(tm_mesh& M, const double* d_src, int width, double* host)
     launch_gather_rows(d_src, M.d_iperm.p, M.stage.p, M.nv, width, g_rt.stream);
     check_launch();
     d2h(host, M.stage.p, sizeof(double) * width * M.nv);
+}
+
+void pull_points(tm_mesh& M)
+{
+    download_rows(M, M.x.p, 3, M.points);
+    sync();
+    M.device_ahead = false;
 }
 
 void refresh_host_topology(tm_mesh& M)
@@ -723,7 +745,7 @@ double* tm_mesh_device_points(tm_mesh* mesh)
 int tm_mesh_upload_points(tm_mesh* mesh)
 {
     API_BEGIN
-    upload_points(*mesh);
+    push_points(*mesh);
     sync();
     API_END
 }
@@ -732,8 +754,14 @@ int tm_mesh_download_points(tm_mesh* mesh)
 {
     API_BEGIN
     ensure_device(*mesh);
-    download_rows(*mesh, mesh->x.p, 3, mesh->points);
-    sync();
+    pull_points(*mesh);
+    API_END
+}
+
+int tm_mesh_sync_host(tm_mesh* mesh)
+{
+    API_BEGIN
+    if (mesh->device_ahead) pull_points(*mesh);
     API_END
 }
 
@@ -1098,3 +1126,126 @@ int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const
 }
 
 }    // extern "C"
+
+// ---- device-resident HMC step ----
+
+struct tm_hmc {
+    tm_mesh* mesh = nullptr;
+    tm_energy* em = nullptr;
+    int model = TM_HMC_ENERGY;
+    int n3 = 0;
+    DevBuf<double> x0, p, a, kpart0, kpart1;
+    DevBuf<HmcState> state;
+};
+
+extern "C" {
+
+int tm_hmc_create(tm_mesh* mesh, tm_energy* em, int model, tm_hmc** out)
+{
+    API_BEGIN
+    require(mesh != nullptr && out != nullptr, "null argument");
+    require(model == TM_HMC_ENERGY || model == TM_HMC_HARMONIC, "unknown HMC energy model");
+    require(model != TM_HMC_ENERGY || em != nullptr, "the energy model needs an energy manager");
+    ensure_device(*mesh);
+    if (em) require(em->nlist && em->nlist->nv == mesh->nv, "mesh does not match the energy manager");
+    auto H = std::make_unique<tm_hmc>();
+    H->mesh = mesh;
+    H->em = model == TM_HMC_ENERGY ? em : nullptr;
+    H->model = model;
+    H->n3 = 3 * mesh->nv;
+    const size_t n3 = std::max(H->n3, 1);
+    H->x0.alloc(n3);
+    H->p.alloc(n3);
+    H->a.alloc(n3);
+    H->kpart0.alloc(hmc_blocks(H->n3));
+    H->kpart1.alloc(hmc_blocks(H->n3));
+    H->state.alloc(1);
+    HIPCHECK(hipMemsetAsync(H->state.p, 0, sizeof(HmcState), g_rt.stream));
+    sync();
+    *out = H.release();
+    API_END
+}
+
+int tm_hmc_destroy(tm_hmc* hmc)
+{
+    delete hmc;
+    return TM_OK;
+}
+
+int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* prm, double temperature, const double* momenta, double uniform,
+                uint64_t seed, uint64_t step)
+{
+    API_BEGIN
+    require(hmc != nullptr && prm != nullptr, "null argument");
+    require(prm->num_integration_steps >= 0, "num_integration_steps must be >= 0");
+    tm_mesh& M = *hmc->mesh;
+    require(3 * M.nv == hmc->n3, "mesh size changed since the HMC handle was created");
+    // the device positions are the state; the host array is taken only when
+    // it is the newer copy
+    if (!M.device_ahead) push_points(M);
+    const int n3 = hmc->n3;
+    const double mass = prm->mass, dt = prm->time_step;
+    const double dt2 = prm->time_step_sq != 0.0 ? prm->time_step_sq : dt * dt;
+    const int L = prm->num_integration_steps;
+    const int harmonic = hmc->model == TM_HMC_HARMONIC;
+    if (momenta && M.nv > 0) {
+        h2d(M.stage.p, momenta, sizeof(double) * n3);
+        launch_gather_rows(M.stage.p, M.d_perm.p, hmc->p.p, M.nv, 3, g_rt.stream);
+        check_launch();
+        sync();    // the caller's array may be reused on return
+    }
+    // p_var = m T (hmc.py:137); sigma = sqrt(p_var) (hmc.py:140)
+    launch_hmc_begin(n3, M.x.p, hmc->x0.p, hmc->p.p, M.d_perm.p, momenta ? 0 : 1, seed, step,
+                     std::sqrt(mass * temperature), hmc->kpart0.p, g_rt.stream);
+    check_launch();
+    HmcState* st = hmc->state.p;
+    const double* energy = nullptr;
+    const double* g = M.x.p;    // harmonic: dE/dx = x
+    if (!harmonic) {
+        tm_energy& E = *hmc->em;
+        ensure_energy_buffers(E, M);
+        energy = &E.globals.p->energy;
+        g = E.grad.p;
+    }
+    // force at x0 (hmc.py:19), then L drift / force / kick steps (hmc.py:20-24)
+    for (int i = 0; i <= L; i++) {
+        if (!harmonic) {
+            run_pass1(*hmc->em, M, hmc->em->precision, nullptr);
+            run_pass2(*hmc->em, M, hmc->em->precision);
+        }
+        int mode = i == 0 ? kHmcFirst : kHmcKick;
+        mode |= i < L ? kHmcDrift : kHmcLast;
+        launch_hmc_leap(n3, M.x.p, hmc->p.p, hmc->a.p, g, dt, dt2, mass, mode, energy, st, hmc->kpart1.p,
+                        g_rt.stream);
+        check_launch();
+    }
+    launch_hmc_accept(n3, hmc->kpart0.p, hmc->kpart1.p, hmc->x0.p, M.x.p, harmonic, energy, mass, temperature,
+                      uniform, seed, step, st, g_rt.stream);
+    check_launch();
+    launch_hmc_restore(n3, M.x.p, hmc->x0.p, st, g_rt.stream);
+    check_launch();
+    M.device_ahead = true;
+    API_END
+}
+
+int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out)
+{
+    API_BEGIN
+    require(hmc != nullptr && out != nullptr, "null argument");
+    HmcState st;
+    d2h(&st, hmc->state.p, sizeof(HmcState));
+    sync();
+    out->dh = st.dh;
+    out->energy0 = st.energy0;
+    out->energy1 = st.energy1;
+    out->kinetic0 = st.kinetic0;
+    out->kinetic1 = st.kinetic1;
+    out->uniform = st.uniform;
+    out->accepted = st.accepted;
+    out->n_accepted = (long long)st.n_accepted;
+    out->n_steps = (long long)st.n_steps;
+    API_END
+}
+
+}    // extern "C"
+

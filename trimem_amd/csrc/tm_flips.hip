@@ -15,6 +15,7 @@
 // caller or drawn from a counter-based generator keyed by a seed.
 #include "tm_internal.h"
 #include "tm_om.h"
+#include "tm_rng.h"
 
 #include <algorithm>
 
@@ -68,22 +69,7 @@ void launch_gather_rows(const double* src, const int* idx, double* dst, int n, i
 }
 
 // ---------------------------------------------------------------- RNG -----
-__device__ __forceinline__ uint64_t splitmix64(uint64_t z)
-{
-    z += 0x9e3779b97f4a7c15ull;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-// stream 0: proposals, stream 1: uniforms
-__device__ __forceinline__ uint64_t rng_draw(uint64_t seed, uint64_t stream, uint64_t i)
-{
-    return splitmix64(splitmix64(seed ^ (stream * 0xd1b54a32d192ed03ull)) + i);
-}
-__device__ __forceinline__ double rng_uniform(uint64_t seed, uint64_t i)
-{
-    return (double)(rng_draw(seed, 1, i) >> 11) * (1.0 / 9007199254740992.0);
-}
+__device__ __forceinline__ double rng_uniform(uint64_t seed, uint64_t i) { return rng_unit(rng_draw(seed, 1, i)); }
 __device__ __forceinline__ int rng_edge(uint64_t seed, uint64_t i, int ne)
 {
     return (int)(((rng_draw(seed, 0, i) >> 32) * (uint64_t)ne) >> 32);

@@ -76,4 +76,21 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
                           uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
                           double* state, hipStream_t s);
 
+// device HMC step (tm_hmc.hip)
+struct HmcState {
+    double energy0, energy1, kinetic0, kinetic1, dh, uniform;
+    int accepted, pad;
+    unsigned long long n_accepted, n_steps;
+};
+enum { kHmcFirst = 1, kHmcKick = 2, kHmcDrift = 4, kHmcLast = 8 };
+int hmc_blocks(int n3);
+void launch_hmc_begin(int n3, const double* x, double* x0, double* p, const int* perm, int draw, uint64_t seed,
+                      uint64_t step, double sigma, double* kpart, hipStream_t s);
+void launch_hmc_leap(int n3, double* x, double* p, double* a, const double* g, double dt, double dt2, double mass,
+                     int mode, const double* energy, HmcState* st, double* kpart, hipStream_t s);
+void launch_hmc_accept(int n3, const double* kpart0, const double* kpart1, const double* x0, const double* x,
+                       int harmonic, const double* energy, double mass, double temperature, double uniform,
+                       uint64_t seed, uint64_t step, HmcState* st, hipStream_t s);
+void launch_hmc_restore(int n3, double* x, const double* x0, const HmcState* st, hipStream_t s);
+
 }    // namespace tmh
