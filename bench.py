@@ -104,6 +104,81 @@ def cpu_baseline(v, f, params, budget_s):
     return 1.0 / float(np.median(times)), len(times), O.num_threads()
 
 
+def core_params(core, params):
+    """trimem_amd.core.EnergyParams of a workload() parameter dict."""
+    e = core.EnergyParams()
+    for k in ("kappa_b", "kappa_a", "kappa_v", "kappa_c", "kappa_t", "kappa_r",
+              "area_frac", "volume_frac", "curvature_frac"):
+        setattr(e, k, params[k])
+    b = core.BondParams()
+    b.type = [core.BondType.Edge, core.BondType.Area][params["bond_type"]]
+    b.r, b.lc0, b.lc1, b.a0 = params["bond_r"], params["bond_lc0"], params["bond_lc1"], params["bond_a0"]
+    e.bond_params = b
+    r = core.SurfaceRepulsionParams()
+    r.n_search = ["cell-list", "verlet-list"][params["rep_nlist"]]
+    r.rlist, r.lc1, r.r, r.exclusion_level = (params["rep_rlist"], params["rep_lc1"], params["rep_r"],
+                                              params["rep_exclusion"])
+    e.repulse_params = r
+    c = core.ContinuationParams()
+    c.delta, c.lam = params["cont_delta"], params["cont_lambda"]
+    e.continuation_params = c
+    return e
+
+
+def hmc_rates(v, f, params, precision, dev_steps, host_steps):
+    """HMC steps/s of the same chain two ways: the device-resident step
+    (DeviceMeshHMC, device draws) and the drop-in path (MeshHMC over
+    EnergyEvaluators over trimem_amd.core, numpy draws, positions through
+    the host view on every force call).  Both: dt 1e-5, 10 leapfrog steps,
+    mass 1, T 1, neighbour list rebuilt after every step."""
+    import contextlib
+    import io
+    from trimem_amd import core
+    from trimem_amd.mc import evaluators as EV
+    from trimem_amd.mc import hmc as H
+    from trimem_amd.mc.mesh import Mesh
+
+    opts = {"time_step": 1e-5, "num_integration_steps": 10, "info_step": 0}
+    quiet = contextlib.redirect_stdout(io.StringIO())
+    mesh = Mesh(v, f)
+    est = core.EnergyManager(mesh.trimesh, core_params(core, params))
+    est.precision = precision
+    with quiet:
+        dev = H.DeviceMeshHMC(mesh, est, counter=H.get_step_counters(), options=opts, rng="device", seed=1)
+    dev.step()
+    est.update_repulsion(mesh.trimesh)
+    a0 = dev.result()["n_accepted"]
+    t0 = time.perf_counter()
+    for _ in range(dev_steps):
+        dev.step()
+        est.update_repulsion(mesh.trimesh)
+    r = dev.result()
+    t_dev = time.perf_counter() - t0
+
+    mesh2 = Mesh(v, f)
+    est2 = core.EnergyManager(mesh2.trimesh, core_params(core, params))
+    est2.precision = precision
+    counter = H.get_step_counters()
+    with contextlib.redirect_stdout(io.StringIO()):
+        funcs = EV.EnergyEvaluators(mesh2, est2, None, {"info_step": 0, "output_step": 0, "refresh_step": 1})
+        host = H.MeshHMC(mesh2, funcs.fun, funcs.grad, counter=counter, options=opts)
+    host.x = mesh2.x.copy()
+    np.random.seed(1)
+    host.step()
+    funcs.callback(host.x, counter)
+    t0 = time.perf_counter()
+    for _ in range(host_steps):
+        host.step()
+        funcs.callback(host.x, counter)
+    t_host = time.perf_counter() - t0
+    return {"device_steps_per_s": dev_steps / t_dev, "dropin_steps_per_s": host_steps / t_host,
+            "device_over_dropin": (dev_steps / t_dev) / (host_steps / t_host),
+            "device_acceptance": (r["n_accepted"] - a0) / dev_steps,
+            "device_steps": dev_steps, "dropin_steps": host_steps,
+            "trajectory": "dt 1e-5, 10 leapfrog steps (11 force evaluations), mass 1, T 1, "
+                          "neighbour list rebuilt every step"}
+
+
 def _csv_rows(path):
     import csv
     return list(csv.DictReader(open(path)))
@@ -120,17 +195,23 @@ def _find(d, suffix):
 def _short(name):
     if "copyBuffer" in name:
         return "copy"
-    for k in ("k_pass1", "k_pass2", "k_finalize", "k_generic_props", "k_generic_grad"):
+    for k in ("k_rep", "k_pass1", "k_pass2", "k_finalize", "k_generic_props", "k_generic_grad"):
         if k in name:
             return k + ("_fp64" if "<double>" in name else "")
     return None
 
 
-def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command):
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
+               "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU")
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4    # 1024 SIMDs, one wave64 VALU op per 4 cycles at 2.4 GHz
+
+
+def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir=None):
     """Summaries of rocprofv3 runs of this script: average kernel durations
     (--kernel-trace --stats) and HBM KiB per dispatch (--pmc FETCH_SIZE, then
     WRITE_SIZE), scaled by what the counters read for the calibration copy
-    of COPY_BYTES."""
+    of COPY_BYTES; with sq_dir, the VALU instructions per dispatch
+    (--pmc SQ_COUNTERS) and the time their issue takes."""
     stats = {}
     for r in _csv_rows(_find(trace_dir, "kernel_stats.csv")):
         k = _short(r["Name"])
@@ -158,6 +239,18 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command):
         kernels[k] = {"fetch_kib": fk, "write_kib": wk,
                       "hbm_bytes_per_dispatch": (fk / f_fetch + wk / f_write) * 1024.0,
                       "avg_us": stats.get(k, {}).get("avg_us")}
+    if sq_dir:
+        sq = {}
+        for r in _csv_rows(_find(sq_dir, "counter_collection.csv")):
+            k = _short(r.get("Kernel_Name", r.get("Kernel-Name", "")))
+            if k and k != "copy":
+                sq.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        for k, c in sq.items():
+            if k in kernels and "SQ_INSTS_VALU" in c:
+                valu = float(np.mean(c["SQ_INSTS_VALU"]))
+                kernels[k]["valu_insts"] = valu
+                kernels[k]["valu_issue_us"] = valu / VALU_ISSUE_PER_S * 1e6
+                kernels[k]["waves"] = float(np.mean(c.get("SQ_WAVES", [0])))
     out = {"command": command, "copy_bytes": COPY_BYTES, "copy_factor_fetch": f_fetch,
            "copy_factor_write": f_write, "kernels": kernels, "durations": stats}
     json.dump(out, open(out_json, "w"), indent=1)
@@ -172,6 +265,22 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command):
         lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
                      f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
                      f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
+    p1 = [k for k in ("k_rep", "k_pass1", "k_finalize") if k in stats]
+    if p1:
+        lines += ["", f"Pass 1 ({' + '.join(p1)}): {sum(stats[k]['avg_us'] for k in p1):.2f} us, "
+                      f"{sum(kernels.get(k, {}).get('hbm_bytes_per_dispatch', 0.0) for k in p1):.0f} HBM bytes."]
+    if sq_dir:
+        lines += ["", "VALU issue (--pmc " + " ".join(SQ_COUNTERS) + "): wave64 VALU instructions per dispatch "
+                  "and the time their issue takes at one instruction per 4 cycles on each of 1024 SIMDs "
+                  "at 2.4 GHz.", "",
+                  "| kernel | waves | VALU insts | VALU insts / wave | VALU issue (us) | issue / duration |",
+                  "|---|---|---|---|---|---|"]
+        for k in sorted(kernels):
+            kk = kernels[k]
+            if "valu_insts" in kk and stats.get(k):
+                lines.append(f"| {k} | {kk['waves']:.0f} | {kk['valu_insts']:.0f} | "
+                             f"{kk['valu_insts'] / max(kk['waves'], 1):.0f} | {kk['valu_issue_us']:.2f} | "
+                             f"{kk['valu_issue_us'] / stats[k]['avg_us']:.2f} |")
     open(out_md, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
@@ -185,19 +294,26 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
-                    help="write profiles/ summaries from three rocprofv3 output folders and exit")
+    ap.add_argument("--hmc-steps", type=int, default=40,
+                    help="device-resident HMC steps timed for the hmc block (0: skip)")
+    ap.add_argument("--summarize", nargs="+", metavar="DIR",
+                    help="TRACE_DIR FETCH_DIR WRITE_DIR [SQ_DIR]: write profiles/ summaries from the "
+                         "rocprofv3 output folders and exit")
     args = ap.parse_args()
     if args.summarize:
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
-               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline",
+               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0",
                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + args.summarize[1] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline",
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0",
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + args.summarize[2] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline",
-               "python3 bench.py --summarize " + " ".join(args.summarize)]
-        summarize(*args.summarize, PROFILE_JSON, os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"),
-                  cmd)
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline"]
+        if len(args.summarize) > 3:
+            cmd.append("rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d " +
+                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0")
+        cmd.append("python3 bench.py --summarize " + " ".join(args.summarize))
+        summarize(args.summarize[0], args.summarize[1], args.summarize[2], PROFILE_JSON,
+                  os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"), cmd,
+                  args.summarize[3] if len(args.summarize) > 3 else None)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: every box here has a single MI355X; "
@@ -260,23 +376,28 @@ def main():
     capi.check(L.tm_energy_eval_device(em, mesh, both))
     capi.check(L.tm_sync())
 
-    # algorithmic bytes per vertex: pass 1 reads positions (24),
-    # ring ids (4 per neighbour, 2E/N = 6 on a closed mesh), meta (4) and
-    # writes q (8); pass 2 reads positions, ring ids, meta and q of the ring
-    # and writes the gradient (24).
-    ring_b = 4.0 * 2 * (3 * n - 6) / n if n > 0 else 0.0
-    b_pass1 = n * (24 + ring_b + 4 + 8)
-    b_pass2 = n * (24 + ring_b + 4 + 8 + 24)
+    # algorithmic bytes per launch: pass 1 reads positions (24 B/vertex) and
+    # the triangle table (12 B/face) and writes a_i, c_i (16 B/vertex); pass 2
+    # reads the same plus a_i, c_i and writes the gradient (24 B/vertex); the
+    # positions are FP64 in both precision paths
+    b_pass1 = 24.0 * n + 12.0 * nf + 16.0 * n
+    b_pass2 = b_pass1 + 24.0 * n
     t1, t2 = float(np.mean(p1)) * 1e-3, float(np.mean(p2)) * 1e-3
-    dom = ("k_pass2", b_pass2, t2) if t2 >= t1 else ("k_pass1", b_pass1, t1)
+    # pass 1 is three launches on the stream (repulsion query, ring
+    # properties, fixed-order global sums), timed together; pass 2 is one
+    phases = {"pass1": ("k_rep", "k_pass1", "k_finalize"), "pass2": ("k_pass2",)}
+    dom = ("pass2", b_pass2, t2) if t2 >= t1 else ("pass1", b_pass1, t1)
     achieved = dom[1] / dom[2] / 1e9
     eval_bytes = b_pass1 + b_pass2
-    traffic = None
+    traffic, valu_frac = None, None
     if os.path.exists(PROFILE_JSON) and args.level == 7 and args.precision == "mixed":
         prof = json.load(open(PROFILE_JSON))
-        k = prof["kernels"].get(dom[0])
-        if k is not None:
-            traffic = k["hbm_bytes_per_dispatch"]
+        ks_ = [prof["kernels"].get(k) for k in phases[dom[0]]]
+        if all(k is not None for k in ks_):
+            traffic = sum(k["hbm_bytes_per_dispatch"] for k in ks_)
+            if all("valu_issue_us" in k for k in ks_):
+                # time the phase's VALU instructions take to issue / its duration
+                valu_frac = sum(k["valu_issue_us"] for k in ks_) / (dom[2] * 1e6)
 
     out = {
         "metric": "energy+grad evals/s",
@@ -297,12 +418,15 @@ def main():
                                "edge tether, cell-list repulsion excl 2 rc=1.5l; one step = energy+gradient",
                    "precision": args.precision, "vertices": n, "faces": nf},
         "hbm_gbps_per_eval": eval_bytes * evals_per_s / 1e9,
-        "kernels_ms": {"k_pass1": float(np.mean(p1)), "k_pass2": float(np.mean(p2))},
-        "roofline": {"kernel": dom[0], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "kernels_ms": {"pass1": float(np.mean(p1)), "pass2": float(np.mean(p2))},
+        "roofline": {"kernel": dom[0] + " (" + " + ".join(phases[dom[0]]) + ")", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "valu_issue_frac": valu_frac,
                      "algorithmic_bytes": dom[1]},
         "wall_s_timed": wall,
     }
+    if args.hmc_steps > 0:
+        out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))
     if not args.no_cpu_baseline:
         cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",

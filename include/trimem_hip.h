@@ -92,9 +92,9 @@ double* tm_mesh_device_points(tm_mesh* mesh);
 int tm_mesh_upload_points(tm_mesh* mesh);
 int tm_mesh_download_points(tm_mesh* mesh);
 /* After a device-resident step (tm_hmc_step) the device positions are ahead
- * of the host array; this copies them back if so (no-op otherwise).  Every
- * host-facing compute call does the same before it reads positions, instead
- * of uploading the stale host array. */
+ * of the host array: compute calls then use the device copy as it is, and
+ * this call copies it back to the host array (no-op otherwise).  Call it
+ * before reading or writing tm_mesh_points after device-resident steps. */
 int tm_mesh_sync_host(tm_mesh* mesh);
 
 /* ---- geometry on one halfedge (mesh_util.h:15-185, bindings.cpp:244-407) ----

@@ -639,6 +639,7 @@ PYBIND11_MODULE(core, m)
     m.def(
         "gradient",
         [](TriMesh& t, EnergyManager& e, py::array_t<double> grad, double eps) {
+            check(tm_mesh_sync_host(t.h));
             double e0 = e.energy(t);
             auto g = grad.mutable_unchecked<2>();
             double* pts = tm_mesh_points(t.h);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -339,17 +340,14 @@ void push_points(tm_mesh& M)
     check_launch();
 }
 
-void pull_points(tm_mesh& M);
-
-// Host-facing calls read positions from the host array (the reference's
+// Host-facing calls take positions from the host array (the reference's
 // points() view, mesh_py.cpp:96-105).  When a device-resident step has moved
-// the device copy since, the device is the newer state: copy it back to the
-// host array instead of overwriting it.
+// the device copy since, the device holds the newer state and is used as it
+// is; the host array is refreshed only when it is read (tm_mesh_sync_host).
 void upload_points(tm_mesh& M)
 {
     ensure_device(M);
-    if (M.device_ahead) pull_points(M);
-    else push_points(M);
+    if (!M.device_ahead) push_points(M);
 }
 
 // rows of an internal-order device array -> user order in the stage buffer
@@ -391,6 +389,8 @@ struct tm_nlist {
     double rlist = 0.1;
     NListView v{};
     DevBuf<int> cell_start, cell_verts, vert_cell, counts, fill, scan_tmp, adj_start, adj, adj_counts;
+    DevBuf<int> cand_count, cand_start, cand;
+    DevBuf<int2> vcand;
     // distance-matrix scratch
     DevBuf<int> dm_counts, dm_off;
     DevBuf<double> dm_d;
@@ -466,6 +466,25 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     v.cell_verts = L.cell_verts.p;
     v.vert_cell = L.vert_cell.p;
     v.perm = M.d_perm.p;
+    if (L.kind == 0 && n > 0) {
+        if ((long long)27 * n > INT_MAX) throw TmError(TM_E_UNSUPPORTED, "mesh too large for the candidate table");
+        L.cand_count.alloc(n + 1);
+        L.cand_start.alloc(n + 1);
+        HIPCHECK(hipMemsetAsync(L.cand_count.p + n, 0, sizeof(int), g_rt.stream));
+        launch_cand_count(v, n, L.cand_count.p, g_rt.stream);
+        check_launch();
+        scan(L.cand_count.p, L.cand_start.p, n + 1, L.scan_tmp);
+        // a vertex is in the stencil lists of at most 27 cells: 27 N bounds
+        // the total without reading it back
+        L.cand.alloc((size_t)27 * n);
+        launch_cand_fill(v, n, L.cand_start.p, L.cand.p, g_rt.stream);
+        check_launch();
+        L.vcand.alloc(n);
+        launch_cand_vrange(v, n, L.cand_start.p, L.vcand.p, g_rt.stream);
+        check_launch();
+        v.vcand = L.vcand.p;
+        v.cand = L.cand.p;
+    }
     if (L.kind == 1) {
         L.adj_counts.alloc(n + 1);
         L.adj_start.alloc(n + 1);
@@ -516,6 +535,7 @@ struct tm_energy {
     int nv = 0;
     DevBuf<double> q, partials, grad, vprops, gprops;
     DevBuf<int> pair_cnt;
+    DevBuf<double> rep_v;
     DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<int> flip_result, proposals, pf_status;
@@ -551,6 +571,7 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     E.gprops.alloc((size_t)M.nv * 6);
     E.pair_cnt.alloc(M.nv);
     HIPCHECK(hipMemsetAsync(E.pair_cnt.p, 0, sizeof(int) * M.nv, g_rt.stream));
+    E.rep_v.alloc(M.nv);
     E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
     E.grad.alloc((size_t)M.nv * 3);
     E.counter.alloc(2);
@@ -565,6 +586,7 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.q = E.q.p;
     b.gprops = E.gprops.p;
     b.pair_cnt = E.pair_cnt.p;
+    b.rep_v = E.rep_v.p;
     b.partials = E.partials.p;
     b.counter = E.counter.p;
     b.globals = E.globals.p;

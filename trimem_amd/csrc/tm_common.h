@@ -24,6 +24,7 @@
 namespace tmh {
 
 constexpr int kMaxValence = 32;    // ring table width W
+constexpr int kRingPad = 8;        // ring columns [valence, kRingPad) hold -1
 
 template <class T>
 struct V3 {
@@ -97,6 +98,10 @@ struct NListView {
     const int* __restrict__ cell_verts;    // vertices sorted by (cell, id)
     const int* __restrict__ vert_cell;     // build-time cell of each vertex
     const int* __restrict__ perm;          // internal -> user vertex id
+    // per occupied cell, the vertices of its 27-cell stencil, stored once
+    // per cell; vcand[v] is the list of the build-time cell of v
+    const int2* __restrict__ vcand;        // [N] range in cand of the build-time cell of each vertex
+    const int* __restrict__ cand;
     // verlet list
     const int* __restrict__ adj_start;     // [N + 1]
     const int* __restrict__ adj;           // neighbour ids, ascending per vertex

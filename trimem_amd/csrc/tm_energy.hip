@@ -146,6 +146,7 @@ __device__ void finalize_globals(const double* __restrict__ partials, int nblock
 // vertex's ring are issued at once instead of as a chain of dependent
 // round trips, and the group sums its per-slot terms with lane shuffles.
 constexpr int kGroup = 8;
+static_assert(kGroup == kRingPad, "exclusion tests read kGroup ring columns without the valence");
 
 // Sums over the 8 lanes of a group with DPP lane moves (no LDS traffic):
 // xor 1 and xor 2 within quads, then each lane i of the group pairs with lane
@@ -250,7 +251,9 @@ __device__ __forceinline__ bool in_ring(const RingRegs& R, int j)
     return hit;
 }
 // Same answer as ring_excluded(m, v, n, j, excl) for n <= kGroup, with the
-// ring of v in registers and the ring of j fetched as independent loads.
+// ring of v in registers.  The first kGroup columns of the ring of j are
+// read without its valence (unused columns hold -1); meta[j] and the
+// columns beyond are read only when j has kGroup or more neighbours.
 __device__ __forceinline__ bool excluded_fast(const MeshView& m, const RingRegs& R, int v, int j, int excl)
 {
     if (j == v) return true;
@@ -258,99 +261,89 @@ __device__ __forceinline__ bool excluded_fast(const MeshView& m, const RingRegs&
     if (R.n > kGroup) return ring_excluded(m, v, R.n, j, excl);
     if (in_ring(R, j)) return true;
     if (excl == 1) return false;
-    const int nj = meta_valence(m.meta[j]);
     int w[kGroup];
 #pragma unroll
-    for (int a = 0; a < kGroup; a++) w[a] = a < nj ? m.ring[a * m.nv + j] : -2;
+    for (int a = 0; a < kGroup; a++) w[a] = m.ring[a * m.nv + j];
     bool hit = false;
 #pragma unroll
-    for (int a = 0; a < kGroup; a++) hit |= in_ring(R, w[a]);
+    for (int a = 0; a < kGroup; a++) hit |= w[a] >= 0 && in_ring(R, w[a]);
     if (hit) return true;
-    for (int a = kGroup; a < nj; a++)
-        if (in_ring(R, m.ring[a * m.nv + j])) return true;
+    if (w[kGroup - 1] >= 0) {
+        const int nj = meta_valence(m.meta[j]);
+        for (int a = kGroup; a < nj; a++)
+            if (in_ring(R, m.ring[a * m.nv + j])) return true;
+    }
     return false;
 }
 
 // The neighbour query of point_distances (cell_list.h:133-219,
-// verlet_list.h:76-101) shared by the lanes of a group: lane r scans cell
-// row r of the 3x3 rows of the stencil (lane 0 also row 8), two candidates
-// per step; for the Verlet list the adjacency is strided over the lanes.
-// f(j, d) sees the pairs of this lane; together the lanes see the
-// reference's set exactly once.
-template <class F>
+// verlet_list.h:76-101) shared by the lanes of a group: the candidates (the
+// stored stencil list of the vertex's cell, the stencil rows of its current
+// cell if it has moved to another cell, or its Verlet adjacency) are strided
+// over the lanes.  f(j, d) sees the pairs of this lane; together the lanes
+// see the reference's set exactly once.
+template <int kBatch = 1, class F>
 __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListView& nl, int v, int n, d3 pi,
                                                  double dmax, int lane, F&& f)
 {
+    // list range and build-time cell of v: independent of the ring loads
+    int2 vr = make_int2(0, 0);
+    int vc = -1;
+    if (nl.kind == 0 && nl.vcand) {
+        vr = nl.vcand[v];
+        vc = nl.vert_cell[v];
+    }
     const double dmax2 = dmax * dmax * (1.0 + 1e-15);
     const RingRegs R = load_ring(m, v, n);
     auto test = [&](int j, d3 pj) {
-        d3 d = pi - pj;
-        double sq = sqnorm(d);
+        const d3 d = pi - pj;
+        const double sq = sqnorm(d);
         if (sq <= dmax2 && sqrt(sq) < dmax) {
             if (nl.kind == 1 || !excluded_fast(m, R, v, j, nl.exclusion)) f(j, d);
         }
     };
-    auto scan = [&](const int* ids, int s, int e, int step) {
-        for (int q = s; q < e; q += 2 * step) {
-            int j0 = ids[q];
-            bool two = q + step < e;
-            int j1 = two ? ids[q + step] : j0;
-            d3 p0 = load_p(m.x, j0);
-            d3 p1 = load_p(m.x, j1);
-            test(j0, p0);
-            if (two) test(j1, p1);
+    // kBatch candidates per lane and step: their ids, then their positions,
+    // are all in flight before the first test
+    auto scan = [&](const int* ids, int s, int e) {
+        for (int q0 = s + lane; q0 < e; q0 += kBatch * kGroup) {
+            int j[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const int q = q0 + u * kGroup;
+                j[u] = q < e ? ids[q] : -1;
+            }
+            d3 pj[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) pj[u] = load_p(m.x, j[u] >= 0 ? j[u] : v);
+#pragma unroll
+            for (int u = 0; u < kBatch; u++)
+                if (j[u] >= 0) test(j[u], pj[u]);
         }
     };
     if (nl.kind == 0) {
-        // bounds of the 9 cell rows: lane r loads row r (lane 0 also row 8),
-        // the group shares them; candidates of all rows form one index range
-        // that is strided over the lanes, so the lanes stay balanced
         int c[3];
         cell_coords(nl, pi, c);
-        const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
-        auto row = [&](int r, int& s0, int& len) {
-            int kk = c[2] + r / 3 - 1;
-            int jj = c[1] + r % 3 - 1;
-            s0 = 0;
-            len = 0;
-            if (i0 > i1 || kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) return;
-            int base = jj * nl.strides[1] + kk * nl.strides[2];
-            s0 = nl.cell_start[base + i0];
-            len = nl.cell_start[base + i1 + 1] - s0;
-        };
-        int my_s, my_len, s8, len8;
-        row(lane, my_s, my_len);
-        row(8, s8, len8);
-        const int base_lane = threadIdx.x & ~(kGroup - 1);
-        int rs[9], rl[9];
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            rs[r] = __shfl(my_s, base_lane + r, 64);
-            rl[r] = __shfl(my_len, base_lane + r, 64);
+        // a vertex still in its build-time cell scans that cell's stored
+        // stencil list: the same candidates as the 27-cell walk below
+        const bool inside = c[0] >= 0 && c[0] < nl.shape[0] && c[1] >= 0 && c[1] < nl.shape[1] && c[2] >= 0 &&
+                            c[2] < nl.shape[2];
+        const int flat = c[0] * nl.strides[0] + c[1] * nl.strides[1] + c[2] * nl.strides[2];
+        if (inside && flat == vc) {
+            scan(nl.cand, vr.x, vr.y);
+            return;
         }
-        rs[8] = s8;
-        rl[8] = len8;
-        int total = 0;
-#pragma unroll
-        for (int r = 0; r < 9; r++) total += rl[r];
-        for (int t = lane; t < total; t += 2 * kGroup) {
-            int q0 = -1, q1 = -1;
-            int acc = 0;
-#pragma unroll
-            for (int r = 0; r < 9; r++) {
-                if (t >= acc && t < acc + rl[r]) q0 = rs[r] + (t - acc);
-                if (t + kGroup >= acc && t + kGroup < acc + rl[r]) q1 = rs[r] + (t + kGroup - acc);
-                acc += rl[r];
-            }
-            int j0 = nl.cell_verts[q0];
-            int j1 = q1 >= 0 ? nl.cell_verts[q1] : j0;
-            d3 p0 = load_p(m.x, j0);
-            d3 p1 = load_p(m.x, j1);
-            test(j0, p0);
-            if (q1 >= 0) test(j1, p1);
+        // moved vertex: the 9 rows of x-consecutive cells of the stencil of
+        // its current cell, each strided over the lanes
+        const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
+        if (i0 > i1) return;
+        for (int r = 0; r < 9; r++) {
+            const int kk = c[2] + r / 3 - 1, jj = c[1] + r % 3 - 1;
+            if (kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) continue;
+            const int base = jj * nl.strides[1] + kk * nl.strides[2];
+            scan(nl.cell_verts, nl.cell_start[base + i0], nl.cell_start[base + i1 + 1]);
         }
     } else if (nl.kind == 1) {
-        scan(nl.adj, nl.adj_start[v] + lane, nl.adj_start[v + 1], kGroup);
+        scan(nl.adj, nl.adj_start[v], nl.adj_start[v + 1]);
     }
 }
 
@@ -393,7 +386,7 @@ __device__ __forceinline__ bool is_generic(int meta)
 template <class T>
 __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams p, double* __restrict__ q,
                                                double* __restrict__ vprops, const double* __restrict__ gprops,
-                                               int* __restrict__ pair_cnt, double* __restrict__ partials)
+                                               const double* __restrict__ rep_v, double* __restrict__ partials)
 {
     __shared__ double s_red[4 * 6];
     __shared__ double s_out[6];
@@ -401,30 +394,21 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
+    // repulsion sum of v from k_rep, loaded alongside meta
+    const double rv = (rep_v && active && lane == 0) ? rep_v[v] : 0.0;
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
-    const bool rep_on = ring && p.rep_lc1 > 0 && nl.kind >= 0;
-    double area = 0, vol = 0, curv = 0, teth = 0, rep = 0;
-    int cnt = 0;
+    double area = 0, vol = 0, curv = 0, teth = 0;
     if (ring) {
         const d3 xv = load_p(m.x, v);
         for (int k = lane; k < n; k += kGroup) slot_properties<T>(m, p, v, n, k, xv, area, vol, curv, teth);
-        if (rep_on) {
-            group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, [&](int, d3 d) {
-                rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
-                cnt++;
-            });
-        }
     }
     area = group_sum(area);
     vol = group_sum(vol);
     curv = group_sum(curv);
     teth = group_sum(teth);
-    rep = group_sum(rep);
-    cnt = group_sum(cnt);
     double pr[6] = {0, 0, 0, 0, 0, 0};
     __shared__ double s_q[256 / kGroup];
-    __shared__ int s_cnt[256 / kGroup];
     if (active && lane == 0) {
         if (ring) {
             double a = area / 3, c = curv / 2;
@@ -433,27 +417,65 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
             pr[2] = c;
             pr[3] = 2 * c * c / a;
             pr[4] = teth / (p.bond_type == 0 ? 2 : 3);
-            pr[5] = rep;
+            pr[5] = rv;
         } else {
             for (int k = 0; k < 6; k++) pr[k] = gprops[6 * v + k];
         }
         s_q[threadIdx.x / kGroup] = pr[2] / pr[0];
-        s_cnt[threadIdx.x / kGroup] = ring ? cnt : 0;
         if (vprops) {
             for (int k = 0; k < 6; k++) vprops[6 * v + k] = pr[k];
         }
     }
-    block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q / s_cnt
+    block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q
     {
         // the block's 32 consecutive vertices, written contiguously
         const int v0 = (logical_block() * blockDim.x) / kGroup;
         const int i = threadIdx.x;
-        if (i < 256 / kGroup && v0 + i < m.nv) {
-            q[v0 + i] = s_q[i];
-            pair_cnt[v0 + i] = s_cnt[i];
-        }
+        if (i < 256 / kGroup && v0 + i < m.nv) q[v0 + i] = s_q[i];
     }
     if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = s_out[threadIdx.x];
+}
+
+// Repulsion property of every ring vertex (mesh_repulsion.h:28-54):
+// rep_v[v] = sum of rep(|x_v - x_j|) over the neighbours j of v within lc1,
+// pair_cnt[v] = their number (0 for listed boundary vertices, whose
+// properties k_generic_props computes).  A kernel of its own: the query is a
+// chain of dependent loads (cell, candidate id, position, ring of the
+// candidate) and needs few registers, so it runs at high occupancy, apart
+// from the register-heavy ring work of pass 1.
+constexpr int kRepBatch = 4;
+__global__ void __launch_bounds__(256) k_rep(MeshView m, NListView nl, EParams p, double* __restrict__ rep_v,
+                                             int* __restrict__ pair_cnt)
+{
+    __shared__ double s_rep[256 / kGroup];
+    __shared__ int s_cnt[256 / kGroup];
+    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const int v = t / kGroup, lane = t % kGroup;
+    const bool active = v < m.nv;
+    const int meta = active ? m.meta[v] : 0;
+    const bool ring = active && !is_generic(meta);
+    double rep = 0;
+    int cnt = 0;
+    if (ring) {
+        const d3 xv = load_p(m.x, v);
+        group_neighbours<kRepBatch>(m, nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
+            rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
+            cnt++;
+        });
+    }
+    rep = group_sum(rep);
+    cnt = group_sum(cnt);
+    if (lane == 0) {
+        s_rep[threadIdx.x / kGroup] = rep;
+        s_cnt[threadIdx.x / kGroup] = cnt;
+    }
+    __syncthreads();
+    const int v0 = (logical_block() * blockDim.x) / kGroup;
+    const int i = threadIdx.x;
+    if (i < 256 / kGroup && v0 + i < m.nv) {
+        rep_v[v0 + i] = s_rep[i];
+        pair_cnt[v0 + i] = s_cnt[i];
+    }
 }
 
 // Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
@@ -611,12 +633,13 @@ void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int 
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
     dim3 grid(pass1_blocks(m.nv)), block(256);
+    const bool rep_on = p.rep_lc1 > 0 && nl.kind >= 0;
+    if (rep_on) hipLaunchKernelGGL(k_rep, grid, block, 0, s, m, nl, p, b.rep_v, b.pair_cnt);
+    const double* rep_v = rep_on ? b.rep_v : nullptr;
     if (mixed)
-        hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, b.pair_cnt,
-                           b.partials);
+        hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, rep_v, b.partials);
     else
-        hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, b.pair_cnt,
-                           b.partials);
+        hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, rep_v, b.partials);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, b.partials, (int)grid.x, p, b.globals);
 }
 

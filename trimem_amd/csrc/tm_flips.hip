@@ -43,6 +43,7 @@ __global__ void k_build_ring(MeshView m, int* __restrict__ ring, int* __restrict
         n = kMaxValence;
         b = 0;    // route to the halfedge walker
     }
+    for (int k = n; k < kRingPad; k++) ring[k * N + v] = -1;
     meta[v] = n | ((b + 1) << 16);
 }
 

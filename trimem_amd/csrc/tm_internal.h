@@ -9,7 +9,8 @@ namespace tmh {
 struct PassBuffers {
     double* q;          // [N] curvature / area per vertex
     double* gprops;     // [6N] properties of listed (boundary) vertices
-    int* pair_cnt;      // [N] repulsion pairs found by pass 1
+    int* pair_cnt;      // [N] repulsion pairs per vertex (k_rep)
+    double* rep_v;      // [N] repulsion property per vertex (k_rep)
     double* partials;   // [6 * pass1 blocks]
     unsigned* counter;  // last-block counter of pass 1 (returns to 0)
     Globals* globals;
@@ -43,6 +44,11 @@ void launch_cell_scatter(const int* vert_cell, int n, const int* cell_start, int
                          hipStream_t s);
 void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_start, const int* perm, int* cell_verts,
                                hipStream_t s);
+// per occupied cell, the vertices of its 27-cell stencil (count, then fill
+// after an exclusive scan of the counts)
+void launch_cand_count(const NListView& nl, int n, int* count, hipStream_t s);
+void launch_cand_fill(const NListView& nl, int n, const int* cand_start, int* cand, hipStream_t s);
+void launch_cand_vrange(const NListView& nl, int n, const int* cand_start, int2* vcand, hipStream_t s);
 // distance_matrix (cell list or verlet) in two phases: counts then fill
 void launch_dm_count(const MeshView& m, const NListView& nl, double dmax, int* counts, hipStream_t s);
 void launch_dm_fill(const MeshView& m, const NListView& nl, double dmax, const int* offsets, double* d, int* i,

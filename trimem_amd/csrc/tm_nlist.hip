@@ -115,6 +115,83 @@ void launch_cell_ids(const double* x, int n, const NListView& nl, int* vert_cell
     hipLaunchKernelGGL(k_cell_ids, dim3((n + 255) / 256), dim3(256), 0, s, x, n, nl, vert_cell, counts);
 }
 
+// ------------------------------------------------ stencil candidate lists
+// The 27-cell stencil of cell (c0, c1, c2) as 9 rows of x-consecutive cells,
+// clipped to the grid like the query of cell_list.h:133-219; f(start, len)
+// gets each row's range in cell_verts.
+template <class F>
+__device__ __forceinline__ void for_stencil_rows(const NListView& nl, const int c[3], F&& f)
+{
+    const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
+    if (i0 > i1) return;
+    for (int r = 0; r < 9; r++) {
+        const int kk = c[2] + r / 3 - 1, jj = c[1] + r % 3 - 1;
+        if (kk < 0 || kk >= nl.shape[2] || jj < 0 || jj >= nl.shape[1]) continue;
+        const int base = jj * nl.strides[1] + kk * nl.strides[2];
+        const int s0 = nl.cell_start[base + i0];
+        f(s0, nl.cell_start[base + i1 + 1] - s0);
+    }
+}
+
+__device__ __forceinline__ bool cand_leader(const NListView& nl, int slot, int c[3])
+{
+    const int cf = nl.vert_cell[nl.cell_verts[slot]];
+    if (nl.cell_start[cf] != slot) return false;
+    c[2] = cf / nl.strides[2];
+    c[1] = (cf - c[2] * nl.strides[2]) / nl.strides[1];
+    c[0] = cf - c[2] * nl.strides[2] - c[1] * nl.strides[1];
+    return true;
+}
+
+// count[slot] = stencil size for the first slot of each occupied cell, else 0
+__global__ void k_cand_count(NListView nl, int n, int* __restrict__ count)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    int c[3], total = 0;
+    if (cand_leader(nl, slot, c)) for_stencil_rows(nl, c, [&](int, int len) { total += len; });
+    count[slot] = total;
+}
+
+__global__ void k_cand_fill(NListView nl, int n, const int* __restrict__ cand_start, int* __restrict__ cand)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    int c[3];
+    if (!cand_leader(nl, slot, c)) return;
+    int o = cand_start[slot];
+    for_stencil_rows(nl, c, [&](int s0, int len) {
+        for (int q = 0; q < len; q++) cand[o++] = nl.cell_verts[s0 + q];
+    });
+}
+
+// vcand[v] = list range of the build-time cell of v
+__global__ void k_cand_vrange(NListView nl, int n, const int* __restrict__ cand_start, int2* __restrict__ vcand)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int s = nl.cell_start[nl.vert_cell[v]];
+    vcand[v] = make_int2(cand_start[s], cand_start[s + 1]);
+}
+
+void launch_cand_vrange(const NListView& nl, int n, const int* cand_start, int2* vcand, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cand_vrange, dim3((n + 255) / 256), dim3(256), 0, s, nl, n, cand_start, vcand);
+}
+
+void launch_cand_count(const NListView& nl, int n, int* count, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cand_count, dim3((n + 255) / 256), dim3(256), 0, s, nl, n, count);
+}
+
+void launch_cand_fill(const NListView& nl, int n, const int* cand_start, int* cand, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cand_fill, dim3((n + 255) / 256), dim3(256), 0, s, nl, n, cand_start, cand);
+}
+
 __global__ void k_cell_scatter(const int* __restrict__ vert_cell, int n, const int* __restrict__ cell_start,
                                int* __restrict__ fill, int* __restrict__ cell_verts)
 {
