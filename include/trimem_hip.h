@@ -213,6 +213,52 @@ int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* params, double temperature, co
 /* result of the last step and running counts (synchronises) */
 int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
 
+/* ---- replica groups: batched meshes, parallel tempering, graph replay ----
+ * No counterpart in the reference, which evaluates one mesh per call
+ * (kernel.h:140-262).  R meshes with their energy managers, of any sizes,
+ * advance together: every kernel of an evaluation or of an HMC step is one
+ * launch over all replicas, and each replica's results equal, bit for bit,
+ * those of the single-mesh calls (tm_energy_eval_device, tm_hmc_step with
+ * device draws).  All energy managers must use the same precision. */
+typedef struct tm_replicas tm_replicas;
+int tm_replicas_create(tm_mesh* const* meshes, tm_energy* const* ems, int n_replicas, tm_replicas** out);
+int tm_replicas_destroy(tm_replicas* rs);
+/* TM_EVAL_PROPS | TM_EVAL_GRAD for every replica; results in each energy
+ * manager's device buffers (tm_energy_device_globals / _gradient) */
+int tm_replicas_eval(tm_replicas* rs, int flags);
+/* out[7 r + k]: the six global properties (area, volume, curvature,
+ * bending, tethering, repulsion) and the energy of the last evaluation of
+ * replica r (synchronises) */
+int tm_replicas_globals(tm_replicas* rs, double* out);
+/* gradient of the last evaluation of replica r, double[n_r * 3] in user
+ * vertex order (synchronises) */
+int tm_replicas_gradient(tm_replicas* rs, int r, double* out);
+/* HMC parameters per replica (arrays of n_replicas; seeds key the device
+ * draws, see tm_hmc_step) and the shared number of leapfrog steps */
+int tm_replicas_hmc_setup(tm_replicas* rs, const double* mass, const double* time_step, const double* temperature,
+                          const uint64_t* seeds, int num_integration_steps);
+/* kappa_b ladder (n_replicas values, ascending) for replica exchange: slot
+ * s holds ladder[s]; replica r starts on slot r and is evaluated with the
+ * kappa_b of its current slot */
+int tm_replicas_set_ladder(tm_replicas* rs, const double* ladder, uint64_t seed);
+/* n_steps HMC steps of every replica, momenta and uniforms drawn on the
+ * device from (seed_r, step count of replica r); swap_every > 0: one
+ * exchange round (alternating even and odd slot pairs) after every
+ * swap_every-th step; use_graph != 0: each step replayed from one captured
+ * hipGraph.  Enqueued on the library stream, no host synchronisation. */
+int tm_replicas_hmc_run(tm_replicas* rs, int n_steps, int swap_every, int use_graph);
+/* one exchange round now */
+int tm_replicas_swap(tm_replicas* rs);
+/* per replica: accepted steps, steps, current ladder slot, and the six
+ * global properties of its current state (props: 6 per replica); per slot
+ * s (pair s, s+1): exchanges accepted and tried.  Any pointer may be NULL.
+ * Synchronises. */
+int tm_replicas_stats(tm_replicas* rs, long long* accepted, long long* steps, int* slot_of, double* props,
+                      long long* swaps_accepted, long long* swaps_tried);
+/* rebuild every replica's neighbour list from its current device positions
+ * (EnergyManager.update_repulsion, energy.cpp:54-61) */
+int tm_replicas_update_repulsion(tm_replicas* rs);
+
 #ifdef __cplusplus
 }
 #endif

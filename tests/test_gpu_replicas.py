@@ -1,0 +1,172 @@
+"""GPU: replica groups (trimem_amd.core.Replicas, the tm_replicas_* ABI).
+
+Pinned by per-replica equality with single-mesh runs:
+  * one batched evaluation of meshes of different sizes, parameters,
+    neighbour lists and a boundary mesh gives, for every replica, the same
+    energy and gradient bits as EnergyManager.energy / gradient on that mesh
+    alone (FP64 and mixed);
+  * grouped HMC steps with device draws give the same positions and
+    acceptance counts as DeviceHMC on each mesh alone with the same seed and
+    step numbers;
+  * replaying each step from a captured hipGraph gives the same bits as
+    direct launches;
+  * replica exchange: every swap decision equals the one recomputed on the
+    host from the replicas' bending B, the ladder and the uniform of the
+    counter-based stream (restated below from its definition), and after a
+    swap each replica's energy changes by (new kappa_b - old kappa_b) * B.
+"""
+import numpy as np
+import pytest
+
+from trimem_amd import meshes as M
+
+pytestmark = pytest.mark.gpu
+
+MASK = (1 << 64) - 1
+
+
+def splitmix64(z):
+    z = (z + 0x9E3779B97F4A7C15) & MASK
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+def rng_unit(seed, stream, i):
+    b = splitmix64((splitmix64(seed ^ ((stream * 0xD1B54A32D192ED03) & MASK)) + i) & MASK)
+    return (b >> 11) * (1.0 / 9007199254740992.0)
+
+
+@pytest.fixture(scope="module")
+def core():
+    import trimem_amd.core as m
+    assert m.device_count() > 0, "no GPU visible"
+    return m
+
+
+def params(m, lbar, kappa_b=1.0, vfrac=0.9, n_search="cell-list", excl=0, kappa_r=1.0):
+    e = m.EnergyParams()
+    e.kappa_b, e.kappa_a, e.kappa_v, e.kappa_c, e.kappa_t, e.kappa_r = kappa_b, 1e3, 1e3, 1.0, 10.0, kappa_r
+    e.area_frac, e.volume_frac, e.curvature_frac = 1.0, vfrac, 1.0
+    b = m.BondParams()
+    b.type, b.r, b.lc0, b.lc1 = m.BondType.Edge, 2, 1.25 * lbar, 0.75 * lbar
+    e.bond_params = b
+    r = m.SurfaceRepulsionParams()
+    r.n_search, r.rlist, r.lc1, r.r, r.exclusion_level = n_search, 1.2 * lbar, 1.2 * lbar, 2, excl
+    e.repulse_params = r
+    return e
+
+
+def membrane(core, level, seed, precision, **kw):
+    v, f, lbar = M.jittered_icosphere(level, seed)
+    mesh = core.TriMesh(v, f)
+    est = core.EnergyManager(mesh, params(core, lbar, **kw))
+    est.precision = precision
+    return mesh, est
+
+
+def mixed_set(core, precision):
+    out = [membrane(core, 2, 1, precision),
+           membrane(core, 4, 2, precision, kappa_b=30.0, vfrac=0.8),
+           membrane(core, 3, 3, precision, n_search="verlet-list", excl=1),
+           membrane(core, 3, 4, precision, excl=2)]
+    v, f = M.tube(length=1.0, radius=0.5, segments=24)
+    lbar = M.mean_edge_length(v, f)
+    tube = core.TriMesh(v, f)
+    est = core.EnergyManager(tube, params(core, lbar))
+    est.precision = precision
+    out.append((tube, est))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_eval_equals_single(core, precision):
+    reps = mixed_set(core, precision)
+    R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+    R.eval()
+    g = R.globals()
+    grads = [R.gradient(i) for i in range(len(reps))]
+    for i, (m, e) in enumerate(reps):
+        e_single = e.energy(m)
+        g_single = e.gradient(m)
+        print(precision, i, m.n_vertices(), g[i, 6], e_single, flush=True)
+        assert g[i, 6] == e_single
+        assert np.array_equal(grads[i], g_single)
+
+
+def test_grouped_hmc_equals_single(core):
+    seeds = [11, 22, 33]
+    steps = 5
+    opts = dict(mass=1.0, time_step=2e-3, temperature=1.0, num_integration_steps=8)
+    reps = [membrane(core, 2, 5, "mixed"), membrane(core, 3, 6, "mixed"), membrane(core, 3, 7, "mixed", excl=2)]
+    R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+    R.hmc_setup(seeds=seeds, **opts)
+    R.hmc_run(steps)
+    st = R.stats()
+    xs = [m.points().copy() for m, _ in reps]
+
+    single = [membrane(core, 2, 5, "mixed"), membrane(core, 3, 6, "mixed"), membrane(core, 3, 7, "mixed", excl=2)]
+    for i, (m, e) in enumerate(single):
+        d = core.DeviceHMC(m, e)
+        for k in range(steps):
+            d.step(mass=opts["mass"], time_step=opts["time_step"], temperature=opts["temperature"],
+                   num_integration_steps=opts["num_integration_steps"], seed=seeds[i], step=k)
+        r = d.result()
+        print(i, "accepted grouped", st["accepted"][i], "single", r["n_accepted"], flush=True)
+        assert st["accepted"][i] == r["n_accepted"] and st["steps"][i] == steps
+        assert np.array_equal(m.points(), xs[i])
+
+
+def test_graph_replay_equals_direct(core):
+    out = []
+    for use_graph in (False, True):
+        reps = [membrane(core, 3, 8, "fp64"), membrane(core, 2, 9, "fp64")]
+        R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+        R.hmc_setup(mass=1.0, time_step=2e-3, temperature=1.0, seeds=[5, 6], num_integration_steps=6)
+        R.set_ladder([1.0, 2.0], seed=3)
+        R.hmc_run(6, swap_every=2, use_graph=use_graph)
+        st = R.stats()
+        out.append(([m.points().copy() for m, _ in reps], st["accepted"].copy(), st["slot_of"].copy()))
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def test_exchange_rule(core):
+    # close rungs (swaps mostly accepted) and one wide rung (mostly rejected)
+    ladder = [10.0, 10.05, 10.1, 11.0]
+    seed = 1234
+    reps = [membrane(core, 3, 20 + i, "fp64", kappa_b=ladder[i]) for i in range(4)]
+    R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+    R.hmc_setup(mass=1.0, time_step=1e-3, temperature=1.0, seeds=[1, 2, 3, 4], num_integration_steps=4)
+    R.set_ladder(ladder, seed=seed)
+    R.hmc_run(1)
+    decided = 0
+    for rd in range(6):
+        st = R.stats()
+        slot_of = st["slot_of"].copy()
+        owner = np.argsort(slot_of)
+        B = st["props"][:, 3].copy()
+        R.eval(gradient=False)
+        e_before = R.globals()[:, 6].copy()
+        R.swap()
+        st2 = R.stats()
+        expect = owner.copy()
+        for s in range(rd % 2, 3, 2):
+            a, b = owner[s], owner[s + 1]
+            dE = (ladder[s + 1] - ladder[s]) * (B[a] - B[b])
+            alpha = 1.0 if dE <= 0 else np.exp(-dE)
+            if rng_unit(seed, 5, rd * 4 + s) <= alpha:
+                expect[s], expect[s + 1] = b, a
+            decided += 1
+        print(rd, "B", B, "owner", owner, "->", np.argsort(st2["slot_of"]), flush=True)
+        assert np.array_equal(np.argsort(st2["slot_of"]), expect)
+        R.eval(gradient=False)
+        e_after = R.globals()[:, 6]
+        kb_before = np.array(ladder)[slot_of]
+        kb_after = np.array(ladder)[st2["slot_of"]]
+        assert np.allclose(e_after - e_before, (kb_after - kb_before) * B, rtol=1e-12, atol=1e-9)
+        R.hmc_run(1)
+    assert decided == 9
+    assert st2["swaps_tried"].sum() == 9
+    assert 0 < st2["swaps_accepted"].sum() < 9    # both outcomes exercised

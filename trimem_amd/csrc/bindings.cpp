@@ -387,6 +387,38 @@ struct DeviceHMC {
     }
 };
 
+// R meshes with their energy managers advanced together (tm_replicas_*).
+struct Replicas {
+    tm_replicas* h = nullptr;
+    std::vector<int> nv;
+    Replicas(const std::vector<TriMesh*>& meshes, const std::vector<EnergyManager*>& estores)
+    {
+        if (meshes.size() != estores.size()) throw std::runtime_error("one energy manager per mesh");
+        std::vector<tm_mesh*> m;
+        std::vector<tm_energy*> e;
+        for (size_t i = 0; i < meshes.size(); i++) {
+            m.push_back(meshes[i]->h);
+            e.push_back(estores[i]->h);
+            nv.push_back(meshes[i]->nv());
+        }
+        check(tm_replicas_create(m.data(), e.data(), (int)m.size(), &h));
+    }
+    Replicas(const Replicas&) = delete;
+    ~Replicas() { tm_replicas_destroy(h); }
+    int size() const { return (int)nv.size(); }
+    std::vector<double> per_replica(py::object v, const char* name) const
+    {
+        std::vector<double> out(size());
+        if (py::isinstance<py::float_>(v) || py::isinstance<py::int_>(v)) {
+            std::fill(out.begin(), out.end(), v.cast<double>());
+            return out;
+        }
+        auto a = v.cast<std::vector<double>>();
+        if ((int)a.size() != size()) throw std::runtime_error(std::string(name) + ": one value per replica");
+        return a;
+    }
+};
+
 std::atomic<uint64_t> g_seed_counter{0};
 uint64_t fresh_seed()
 {
@@ -612,6 +644,77 @@ PYBIND11_MODULE(core, m)
             py::arg("temperature"), py::arg("momenta") = py::none(), py::arg("uniform") = -1.0,
             py::arg("seed") = 0, py::arg("step") = 0, py::arg("time_step_sq") = 0.0)
         .def("result", &DeviceHMC::result);
+
+    py::class_<Replicas>(m, "Replicas")
+        .def(py::init([](const std::vector<TriMesh*>& meshes, const std::vector<EnergyManager*>& estores) {
+                 return std::make_unique<Replicas>(meshes, estores);
+             }),
+             py::arg("meshes"), py::arg("estores"), py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def("__len__", &Replicas::size)
+        .def(
+            "eval", [](Replicas& r, bool props, bool grad) {
+                check(tm_replicas_eval(r.h, (props ? TM_EVAL_PROPS : 0) | (grad ? TM_EVAL_GRAD : 0)));
+            },
+            py::arg("properties") = true, py::arg("gradient") = true)
+        .def("globals",
+             [](Replicas& r) {
+                 py::array_t<double> out({(size_t)r.size(), (size_t)7});
+                 check(tm_replicas_globals(r.h, out.mutable_data()));
+                 return out;
+             })
+        .def(
+            "gradient",
+            [](Replicas& r, int i) {
+                if (i < 0 || i >= r.size()) throw py::index_error("replica index out of range");
+                py::array_t<double> out({(size_t)r.nv[i], (size_t)3});
+                check(tm_replicas_gradient(r.h, i, out.mutable_data()));
+                return out;
+            },
+            py::arg("replica"))
+        .def(
+            "hmc_setup",
+            [](Replicas& r, py::object mass, py::object time_step, py::object temperature,
+               std::vector<uint64_t> seeds, int num_integration_steps) {
+                auto ms = r.per_replica(mass, "mass"), dt = r.per_replica(time_step, "time_step"),
+                     T = r.per_replica(temperature, "temperature");
+                if ((int)seeds.size() != r.size()) throw std::runtime_error("seeds: one value per replica");
+                check(tm_replicas_hmc_setup(r.h, ms.data(), dt.data(), T.data(), seeds.data(),
+                                            num_integration_steps));
+            },
+            py::kw_only(), py::arg("mass"), py::arg("time_step"), py::arg("temperature"), py::arg("seeds"),
+            py::arg("num_integration_steps"))
+        .def(
+            "set_ladder",
+            [](Replicas& r, std::vector<double> ladder, uint64_t seed) {
+                if ((int)ladder.size() != r.size()) throw std::runtime_error("ladder: one value per replica");
+                check(tm_replicas_set_ladder(r.h, ladder.data(), seed));
+            },
+            py::arg("ladder"), py::arg("seed") = 0)
+        .def(
+            "hmc_run",
+            [](Replicas& r, int n_steps, int swap_every, bool use_graph) {
+                check(tm_replicas_hmc_run(r.h, n_steps, swap_every, use_graph ? 1 : 0));
+            },
+            py::arg("n_steps"), py::arg("swap_every") = 0, py::arg("use_graph") = false)
+        .def("swap", [](Replicas& r) { check(tm_replicas_swap(r.h)); })
+        .def("stats",
+             [](Replicas& r) {
+                 const size_t R = (size_t)r.size();
+                 py::array_t<long long> acc(R), steps(R), sacc(R), stried(R);
+                 py::array_t<int> slot(R);
+                 py::array_t<double> props({R, (size_t)6});
+                 check(tm_replicas_stats(r.h, acc.mutable_data(), steps.mutable_data(), slot.mutable_data(),
+                                         props.mutable_data(), sacc.mutable_data(), stried.mutable_data()));
+                 py::dict d;
+                 d["accepted"] = acc;
+                 d["steps"] = steps;
+                 d["slot_of"] = slot;
+                 d["props"] = props;
+                 d["swaps_accepted"] = sacc;
+                 d["swaps_tried"] = stried;
+                 return d;
+             })
+        .def("update_repulsion", [](Replicas& r) { check(tm_replicas_update_repulsion(r.h)); });
 
     m.def(
         "make_nlist",

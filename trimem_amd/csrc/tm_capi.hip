@@ -1221,12 +1221,12 @@ int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* prm, double temperature, const
                      std::sqrt(mass * temperature), hmc->kpart0.p, g_rt.stream);
     check_launch();
     HmcState* st = hmc->state.p;
-    const double* energy = nullptr;
+    const Globals* globals = nullptr;
     const double* g = M.x.p;    // harmonic: dE/dx = x
     if (!harmonic) {
         tm_energy& E = *hmc->em;
         ensure_energy_buffers(E, M);
-        energy = &E.globals.p->energy;
+        globals = E.globals.p;
         g = E.grad.p;
     }
     // force at x0 (hmc.py:19), then L drift / force / kick steps (hmc.py:20-24)
@@ -1237,11 +1237,11 @@ int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* prm, double temperature, const
         }
         int mode = i == 0 ? kHmcFirst : kHmcKick;
         mode |= i < L ? kHmcDrift : kHmcLast;
-        launch_hmc_leap(n3, M.x.p, hmc->p.p, hmc->a.p, g, dt, dt2, mass, mode, energy, st, hmc->kpart1.p,
+        launch_hmc_leap(n3, M.x.p, hmc->p.p, hmc->a.p, g, dt, dt2, mass, mode, globals, st, hmc->kpart1.p,
                         g_rt.stream);
         check_launch();
     }
-    launch_hmc_accept(n3, hmc->kpart0.p, hmc->kpart1.p, hmc->x0.p, M.x.p, harmonic, energy, mass, temperature,
+    launch_hmc_accept(n3, hmc->kpart0.p, hmc->kpart1.p, hmc->x0.p, M.x.p, harmonic, globals, mass, temperature,
                       uniform, seed, step, st, g_rt.stream);
     check_launch();
     launch_hmc_restore(n3, M.x.p, hmc->x0.p, st, g_rt.stream);
@@ -1271,3 +1271,339 @@ int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out)
 
 }    // extern "C"
 
+
+// ---- replica groups ----
+struct tm_replicas {
+    int R = 0;
+    int mixed = 0;
+    std::vector<tm_mesh*> meshes;
+    std::vector<tm_energy*> ems;
+    // energy passes
+    std::vector<PassArgs> h_args;
+    DevBuf<PassArgs> d_args;
+    std::vector<int> h_block0;
+    DevBuf<int> d_block0;
+    int total_blocks = 0;
+    // HMC
+    bool hmc_ready = false;
+    int L = 0;
+    std::vector<HmcArgs> h_hmc;
+    DevBuf<HmcArgs> d_hmc;
+    std::vector<int> h_hblock0;
+    DevBuf<int> d_hblock0;
+    int hmc_total = 0;
+    std::vector<DevBuf<double>> x0, p, a, kp0, kp1;
+    DevBuf<HmcState> states;
+    // tempering
+    bool has_ladder = false;
+    uint64_t pt_seed = 0;
+    std::vector<double> h_ladder;
+    DevBuf<double> ladder;
+    DevBuf<int> owner;
+    DevBuf<unsigned long long> pt_round, swaps_acc, swaps_tried;
+    // graph of one HMC step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+    ~tm_replicas() { drop_graph(); }
+};
+
+namespace {
+
+// Rebuild the device table of pass arguments from the replicas' current
+// buffers and neighbour lists.  With a ladder, each replica's kappa_b is that
+// of the slot it holds on the device.
+void refresh_replica_args(tm_replicas& S)
+{
+    std::vector<int> slot_of(S.R);
+    if (S.has_ladder) {
+        std::vector<int> owner(S.R);
+        d2h(owner.data(), S.owner.p, sizeof(int) * S.R);
+        sync();
+        for (int s = 0; s < S.R; s++) slot_of[owner[s]] = s;
+    }
+    S.h_args.resize(S.R);
+    S.h_block0.assign(S.R + 1, 0);
+    for (int r = 0; r < S.R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        tm_energy& E = *S.ems[r];
+        ensure_device(M);
+        ensure_energy_buffers(E, M);
+        EParams ep = eparams_of(E);
+        if (S.has_ladder) ep.kappa_b = S.h_ladder[slot_of[r]];
+        S.h_args[r] = pass_args(M.view(), E.nlist->view(), ep, buffers(E, M), nullptr, E.grad.p);
+        S.h_block0[r + 1] = S.h_block0[r] + S.h_args[r].nblocks;
+    }
+    S.total_blocks = S.h_block0[S.R];
+    S.d_args.alloc(S.R);
+    S.d_block0.alloc(S.R + 1);
+    h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
+    h2d(S.d_block0.p, S.h_block0.data(), sizeof(int) * (S.R + 1));
+    sync();
+}
+
+// every replica's positions on the device (host arrays win where newer)
+void replicas_device_points(tm_replicas& S)
+{
+    for (tm_mesh* M : S.meshes)
+        if (!M->device_ahead) push_points(*M);
+}
+
+void replicas_hmc_step(tm_replicas& S)
+{
+    launch_hmc_begin_grouped(S.d_hmc.p, S.d_hblock0.p, S.R, S.hmc_total, g_rt.stream);
+    for (int i = 0; i <= S.L; i++) {
+        launch_pass1_grouped(S.d_args.p, S.d_block0.p, S.h_args.data(), S.R, S.total_blocks, S.mixed, g_rt.stream);
+        launch_pass2_grouped(S.d_args.p, S.d_block0.p, S.h_args.data(), S.R, S.total_blocks, S.mixed, g_rt.stream);
+        int mode = i == 0 ? kHmcFirst : kHmcKick;
+        mode |= i < S.L ? kHmcDrift : kHmcLast;
+        launch_hmc_leap_grouped(S.d_hmc.p, S.d_hblock0.p, S.R, S.hmc_total, mode, g_rt.stream);
+    }
+    launch_hmc_accept_grouped(S.d_hmc.p, S.R, g_rt.stream);
+    launch_hmc_restore_grouped(S.d_hmc.p, S.d_hblock0.p, S.R, S.hmc_total, g_rt.stream);
+}
+
+}    // namespace
+
+extern "C" {
+
+int tm_replicas_create(tm_mesh* const* meshes, tm_energy* const* ems, int n_replicas, tm_replicas** out)
+{
+    API_BEGIN
+    require(meshes && ems && out && n_replicas > 0, "null argument or no replicas");
+    auto S = std::make_unique<tm_replicas>();
+    S->R = n_replicas;
+    for (int r = 0; r < n_replicas; r++) {
+        require(meshes[r] && ems[r], "null replica");
+        require(ems[r]->nlist && ems[r]->nlist->nv == meshes[r]->nv, "mesh does not match the energy manager");
+        require(ems[r]->precision == ems[0]->precision, "replicas must share one precision");
+        S->meshes.push_back(meshes[r]);
+        S->ems.push_back(ems[r]);
+    }
+    S->mixed = ems[0]->precision;
+    refresh_replica_args(*S);
+    *out = S.release();
+    API_END
+}
+
+int tm_replicas_destroy(tm_replicas* rs)
+{
+    delete rs;
+    return TM_OK;
+}
+
+int tm_replicas_eval(tm_replicas* rs, int flags)
+{
+    API_BEGIN
+    replicas_device_points(*rs);
+    if (flags & TM_EVAL_PROPS)
+        launch_pass1_grouped(rs->d_args.p, rs->d_block0.p, rs->h_args.data(), rs->R, rs->total_blocks, rs->mixed,
+                             g_rt.stream);
+    if (flags & TM_EVAL_GRAD)
+        launch_pass2_grouped(rs->d_args.p, rs->d_block0.p, rs->h_args.data(), rs->R, rs->total_blocks, rs->mixed,
+                             g_rt.stream);
+    check_launch();
+    API_END
+}
+
+int tm_replicas_globals(tm_replicas* rs, double* out)
+{
+    API_BEGIN
+    std::vector<Globals> g(rs->R);
+    for (int r = 0; r < rs->R; r++) d2h(&g[r], rs->ems[r]->globals.p, sizeof(Globals));
+    sync();
+    for (int r = 0; r < rs->R; r++) {
+        for (int k = 0; k < 6; k++) out[7 * r + k] = g[r].props[k];
+        out[7 * r + 6] = g[r].energy;
+    }
+    API_END
+}
+
+int tm_replicas_gradient(tm_replicas* rs, int r, double* out)
+{
+    API_BEGIN
+    require(r >= 0 && r < rs->R, "replica index out of range");
+    download_rows(*rs->meshes[r], rs->ems[r]->grad.p, 3, out);
+    sync();
+    API_END
+}
+
+int tm_replicas_hmc_setup(tm_replicas* rs, const double* mass, const double* time_step, const double* temperature,
+                          const uint64_t* seeds, int num_integration_steps)
+{
+    API_BEGIN
+    require(mass && time_step && temperature && seeds, "null argument");
+    require(num_integration_steps >= 0, "num_integration_steps must be >= 0");
+    tm_replicas& S = *rs;
+    S.drop_graph();
+    S.L = num_integration_steps;
+    const int R = S.R;
+    if (!S.hmc_ready) {
+        S.x0.resize(R);
+        S.p.resize(R);
+        S.a.resize(R);
+        S.kp0.resize(R);
+        S.kp1.resize(R);
+        S.states.alloc(R);
+        HIPCHECK(hipMemsetAsync(S.states.p, 0, sizeof(HmcState) * R, g_rt.stream));
+    }
+    S.h_hmc.resize(R);
+    S.h_hblock0.assign(R + 1, 0);
+    for (int r = 0; r < R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        tm_energy& E = *S.ems[r];
+        const int n3 = 3 * M.nv;
+        const size_t n3a = std::max(n3, 1);
+        S.x0[r].alloc(n3a);
+        S.p[r].alloc(n3a);
+        S.a[r].alloc(n3a);
+        S.kp0[r].alloc(hmc_blocks(n3));
+        S.kp1[r].alloc(hmc_blocks(n3));
+        HmcArgs h{};
+        h.n3 = n3;
+        h.nblocks = hmc_blocks(n3);
+        h.x = M.x.p;
+        h.x0 = S.x0[r].p;
+        h.p = S.p[r].p;
+        h.a = S.a[r].p;
+        h.grad = E.grad.p;
+        h.globals = E.globals.p;
+        h.perm = M.d_perm.p;
+        h.st = S.states.p + r;
+        h.kpart0 = S.kp0[r].p;
+        h.kpart1 = S.kp1[r].p;
+        h.mass = mass[r];
+        h.dt = time_step[r];
+        h.dt2 = time_step[r] * time_step[r];
+        h.temperature = temperature[r];
+        h.seed = seeds[r];
+        S.h_hmc[r] = h;
+        S.h_hblock0[r + 1] = S.h_hblock0[r] + h.nblocks;
+    }
+    S.hmc_total = S.h_hblock0[R];
+    S.d_hmc.alloc(R);
+    S.d_hblock0.alloc(R + 1);
+    h2d(S.d_hmc.p, S.h_hmc.data(), sizeof(HmcArgs) * R);
+    h2d(S.d_hblock0.p, S.h_hblock0.data(), sizeof(int) * (R + 1));
+    sync();
+    S.hmc_ready = true;
+    API_END
+}
+
+int tm_replicas_set_ladder(tm_replicas* rs, const double* ladder, uint64_t seed)
+{
+    API_BEGIN
+    require(ladder != nullptr, "null argument");
+    tm_replicas& S = *rs;
+    S.h_ladder.assign(ladder, ladder + S.R);
+    for (int s = 1; s < S.R; s++) require(S.h_ladder[s] >= S.h_ladder[s - 1], "the ladder must be ascending");
+    S.ladder.alloc(S.R);
+    S.owner.alloc(S.R);
+    S.pt_round.alloc(1);
+    S.swaps_acc.alloc(S.R);
+    S.swaps_tried.alloc(S.R);
+    std::vector<int> own(S.R);
+    for (int s = 0; s < S.R; s++) own[s] = s;
+    h2d(S.ladder.p, S.h_ladder.data(), sizeof(double) * S.R);
+    h2d(S.owner.p, own.data(), sizeof(int) * S.R);
+    HIPCHECK(hipMemsetAsync(S.pt_round.p, 0, sizeof(unsigned long long), g_rt.stream));
+    HIPCHECK(hipMemsetAsync(S.swaps_acc.p, 0, sizeof(unsigned long long) * S.R, g_rt.stream));
+    HIPCHECK(hipMemsetAsync(S.swaps_tried.p, 0, sizeof(unsigned long long) * S.R, g_rt.stream));
+    S.pt_seed = seed;
+    S.has_ladder = true;
+    refresh_replica_args(S);
+    API_END
+}
+
+int tm_replicas_swap(tm_replicas* rs)
+{
+    API_BEGIN
+    tm_replicas& S = *rs;
+    require(S.has_ladder && S.hmc_ready, "exchange needs a ladder and HMC parameters");
+    launch_pt_swap(S.d_args.p, S.d_hmc.p, S.owner.p, S.ladder.p, S.R, S.pt_seed, S.pt_round.p, S.swaps_acc.p,
+                   S.swaps_tried.p, g_rt.stream);
+    check_launch();
+    API_END
+}
+
+int tm_replicas_hmc_run(tm_replicas* rs, int n_steps, int swap_every, int use_graph)
+{
+    API_BEGIN
+    tm_replicas& S = *rs;
+    require(S.hmc_ready, "tm_replicas_hmc_setup first");
+    require(n_steps >= 0, "n_steps must be >= 0");
+    require(swap_every <= 0 || S.has_ladder, "exchange needs a ladder");
+    replicas_device_points(S);
+    if (use_graph && !S.graph_exec && n_steps > 0) {
+        // the step reads every changing quantity (step counts, kappa_b, list
+        // pointers) from device tables, so one capture serves all steps
+        HIPCHECK(hipStreamBeginCapture(g_rt.stream, hipStreamCaptureModeThreadLocal));
+        replicas_hmc_step(S);
+        HIPCHECK(hipStreamEndCapture(g_rt.stream, &S.graph));
+        HIPCHECK(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < n_steps; k++) {
+        if (use_graph) HIPCHECK(hipGraphLaunch(S.graph_exec, g_rt.stream));
+        else replicas_hmc_step(S);
+        check_launch();
+        if (swap_every > 0 && (k + 1) % swap_every == 0)
+            launch_pt_swap(S.d_args.p, S.d_hmc.p, S.owner.p, S.ladder.p, S.R, S.pt_seed, S.pt_round.p,
+                           S.swaps_acc.p, S.swaps_tried.p, g_rt.stream);
+    }
+    check_launch();
+    for (tm_mesh* M : S.meshes) M->device_ahead = true;
+    API_END
+}
+
+int tm_replicas_stats(tm_replicas* rs, long long* accepted, long long* steps, int* slot_of, double* props,
+                      long long* swaps_accepted, long long* swaps_tried)
+{
+    API_BEGIN
+    tm_replicas& S = *rs;
+    const int R = S.R;
+    std::vector<HmcState> st(R);
+    std::vector<int> own(R);
+    std::vector<unsigned long long> sa(R, 0), stt(R, 0);
+    if (S.hmc_ready) d2h(st.data(), S.states.p, sizeof(HmcState) * R);
+    if (S.has_ladder) {
+        d2h(own.data(), S.owner.p, sizeof(int) * R);
+        d2h(sa.data(), S.swaps_acc.p, sizeof(unsigned long long) * R);
+        d2h(stt.data(), S.swaps_tried.p, sizeof(unsigned long long) * R);
+    }
+    sync();
+    for (int r = 0; r < R; r++) {
+        if (accepted) accepted[r] = S.hmc_ready ? (long long)st[r].n_accepted : 0;
+        if (steps) steps[r] = S.hmc_ready ? (long long)st[r].n_steps : 0;
+        if (props)
+            for (int k = 0; k < 6; k++) props[6 * r + k] = S.hmc_ready ? st[r].props[k] : 0.0;
+        if (swaps_accepted) swaps_accepted[r] = (long long)sa[r];
+        if (swaps_tried) swaps_tried[r] = (long long)stt[r];
+    }
+    if (slot_of)
+        for (int s = 0; s < R; s++) slot_of[S.has_ladder ? own[s] : s] = s;
+    API_END
+}
+
+int tm_replicas_update_repulsion(tm_replicas* rs)
+{
+    API_BEGIN
+    tm_replicas& S = *rs;
+    for (int r = 0; r < S.R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        tm_energy& E = *S.ems[r];
+        upload_points(M);
+        tm_nlist* fresh = make_nlist(M, E.params);
+        delete E.nlist;
+        E.nlist = fresh;
+    }
+    refresh_replica_args(S);
+    API_END
+}
+
+}    // extern "C"

@@ -383,19 +383,31 @@ __device__ __forceinline__ bool is_generic(int meta)
 // Pass 1 over all vertices; vertices on the boundary (or with fewer than
 // three neighbours) read the properties that k_generic_props wrote.  Writes
 // q_v, the number of repulsion pairs of v, and the FP64 globals.
-template <class T>
-__global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams p, double* __restrict__ q,
-                                               double* __restrict__ vprops, const double* __restrict__ gprops,
-                                               const double* __restrict__ rep_v, double* __restrict__ partials)
+// replica of logical block lb in a grouped launch (block0 ascending, block0[0] = 0)
+__device__ __forceinline__ int find_replica(const int* __restrict__ block0, int R, int lb)
 {
+    int lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (block0[mid] <= lb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
+{
+    const MeshView& m = A.m;
+    const EParams& p = A.p;
     __shared__ double s_red[4 * 6];
     __shared__ double s_out[6];
-    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const int t = lb * blockDim.x + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
     // repulsion sum of v from k_rep, loaded alongside meta
-    const double rv = (rep_v && active && lane == 0) ? rep_v[v] : 0.0;
+    const double rv = (A.rep_v && active && lane == 0) ? A.rep_v[v] : 0.0;
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
     double area = 0, vol = 0, curv = 0, teth = 0;
@@ -419,21 +431,36 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
             pr[4] = teth / (p.bond_type == 0 ? 2 : 3);
             pr[5] = rv;
         } else {
-            for (int k = 0; k < 6; k++) pr[k] = gprops[6 * v + k];
+            for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
         }
         s_q[threadIdx.x / kGroup] = pr[2] / pr[0];
-        if (vprops) {
-            for (int k = 0; k < 6; k++) vprops[6 * v + k] = pr[k];
+        if (A.vprops) {
+            for (int k = 0; k < 6; k++) A.vprops[6 * v + k] = pr[k];
         }
     }
     block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q
     {
         // the block's 32 consecutive vertices, written contiguously
-        const int v0 = (logical_block() * blockDim.x) / kGroup;
+        const int v0 = (lb * blockDim.x) / kGroup;
         const int i = threadIdx.x;
-        if (i < 256 / kGroup && v0 + i < m.nv) q[v0 + i] = s_q[i];
+        if (i < 256 / kGroup && v0 + i < m.nv) A.q[v0 + i] = s_q[i];
     }
-    if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = s_out[threadIdx.x];
+    if (threadIdx.x < 6) A.partials[lb * 6 + threadIdx.x] = s_out[threadIdx.x];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pass1(PassArgs A)
+{
+    pass1_body<T>(A, logical_block());
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
+                                                       int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    pass1_body<T>(A[r], lb - block0[r]);
 }
 
 // Repulsion property of every ring vertex (mesh_repulsion.h:28-54):
@@ -441,15 +468,17 @@ __global__ void __launch_bounds__(256) k_pass1(MeshView m, NListView nl, EParams
 // pair_cnt[v] = their number (0 for listed boundary vertices, whose
 // properties k_generic_props computes).  A kernel of its own: the query is a
 // chain of dependent loads (cell, candidate id, position, ring of the
-// candidate) and needs few registers, so it runs at high occupancy, apart
-// from the register-heavy ring work of pass 1.
+// candidate) and needs few registers, so it runs apart from the
+// register-heavy ring work of pass 1.
 constexpr int kRepBatch = 4;
-__global__ void __launch_bounds__(256) k_rep(MeshView m, NListView nl, EParams p, double* __restrict__ rep_v,
-                                             int* __restrict__ pair_cnt)
+__device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
 {
+    if (!A.rep_v) return;
+    const MeshView& m = A.m;
+    const EParams& p = A.p;
     __shared__ double s_rep[256 / kGroup];
     __shared__ int s_cnt[256 / kGroup];
-    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const int t = lb * blockDim.x + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
@@ -458,7 +487,7 @@ __global__ void __launch_bounds__(256) k_rep(MeshView m, NListView nl, EParams p
     int cnt = 0;
     if (ring) {
         const d3 xv = load_p(m.x, v);
-        group_neighbours<kRepBatch>(m, nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
+        group_neighbours<kRepBatch>(m, A.nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
             rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
             cnt++;
         });
@@ -470,22 +499,40 @@ __global__ void __launch_bounds__(256) k_rep(MeshView m, NListView nl, EParams p
         s_cnt[threadIdx.x / kGroup] = cnt;
     }
     __syncthreads();
-    const int v0 = (logical_block() * blockDim.x) / kGroup;
+    const int v0 = (lb * blockDim.x) / kGroup;
     const int i = threadIdx.x;
     if (i < 256 / kGroup && v0 + i < m.nv) {
-        rep_v[v0 + i] = s_rep[i];
-        pair_cnt[v0 + i] = s_cnt[i];
+        A.rep_v[v0 + i] = s_rep[i];
+        A.pair_cnt[v0 + i] = s_cnt[i];
     }
+}
+
+__global__ void __launch_bounds__(256) k_rep(PassArgs A) { rep_body(A, logical_block()); }
+
+__global__ void __launch_bounds__(256) k_rep_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
+                                                     int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    rep_body(A[r], lb - block0[r]);
 }
 
 // Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
 // factors (one block; kernel boundaries order it after pass 1).
-__global__ void __launch_bounds__(1024) k_finalize(const double* __restrict__ partials, int nblocks, EParams p,
-                                                   Globals* __restrict__ g)
+__global__ void __launch_bounds__(1024) k_finalize(PassArgs A)
 {
     __shared__ double s_red[16 * 6];
     __shared__ double s_out[6];
-    finalize_globals(partials, nblocks, p, g, s_red, s_out);
+    finalize_globals(A.partials, A.nblocks, A.p, A.globals, s_red, s_out);
+}
+
+// one block per replica
+__global__ void __launch_bounds__(1024) k_finalize_grouped(const PassArgs* __restrict__ A)
+{
+    __shared__ double s_red[16 * 6];
+    __shared__ double s_out[6];
+    const PassArgs& a = A[blockIdx.x];
+    finalize_globals(a.partials, a.nblocks, a.p, a.globals, s_red, s_out);
 }
 
 // OpenMesh-semantics properties of the listed vertices (boundary and
@@ -513,6 +560,30 @@ __global__ void __launch_bounds__(128) k_generic_grad(MeshView m, NListView nl, 
     grad[3 * v] = out.x;
     grad[3 * v + 1] = out.y;
     grad[3 * v + 2] = out.z;
+}
+
+// the same for replica r of a grouped launch, with its parameters read from
+// the device table (parallel tempering changes them there)
+__global__ void __launch_bounds__(128) k_generic_props_at(const PassArgs* __restrict__ A, int r)
+{
+    const PassArgs& a = A[r];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nlist) return;
+    int v = a.list[i];
+    VP res = om_vertex_properties(topo_of(a.m), a.nl, a.p, v);
+    for (int k = 0; k < 6; k++) a.gprops[6 * v + k] = res.v[k];
+}
+
+__global__ void __launch_bounds__(128) k_generic_grad_at(const PassArgs* __restrict__ A, int r)
+{
+    const PassArgs& a = A[r];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nlist) return;
+    int v = a.list[i];
+    d3 out = om_vertex_gradient(topo_of(a.m), a.nl, a.p, *a.globals, a.q, v);
+    a.grad[3 * v] = out.x;
+    a.grad[3 * v + 1] = out.y;
+    a.grad[3 * v + 2] = out.z;
 }
 
 // --------------------------------------------------------------- pass 2 ---
@@ -574,11 +645,12 @@ __device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p,
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_pass2(MeshView m, NListView nl, EParams p, const double* __restrict__ q,
-                                               const Globals* __restrict__ gp, const int* __restrict__ pair_cnt,
-                                               double* __restrict__ grad)
+__device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
-    const int t = logical_block() * blockDim.x + threadIdx.x;
+    const MeshView& m = A.m;
+    const NListView& nl = A.nl;
+    const EParams& p = A.p;
+    const int t = lb * blockDim.x + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
@@ -586,13 +658,13 @@ __global__ void __launch_bounds__(256) k_pass2(MeshView m, NListView nl, EParams
     const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
     d3 acc{0, 0, 0};
     if (ring) {
-        const Globals g = *gp;
+        const Globals g = *A.globals;
         const d3 xv = load_p(m.x, v);
-        const double qv = q[v];
-        for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, q, v, n, k, xv, qv);
+        const double qv = A.q[v];
+        for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, A.q, v, n, k, xv, qv);
         // repulsion (mesh_repulsion.h:56-79, doubled as in
-        // mesh_properties.cpp:138-146) for the vertices pass 1 found pairs for
-        if (p.rep_lc1 > 0 && nl.kind >= 0 && p.kappa_r != 0 && pair_cnt[v] > 0) {
+        // mesh_properties.cpp:138-146) for the vertices k_rep found pairs for
+        if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0) {
             group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, [&](int, d3 d) {
                 acc += (p.kappa_r * 2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d;
             });
@@ -613,12 +685,27 @@ __global__ void __launch_bounds__(256) k_pass2(MeshView m, NListView nl, EParams
         s_ring[r] = ring;
     }
     __syncthreads();
-    const int v0 = (logical_block() * blockDim.x) / kGroup;
+    const int v0 = (lb * blockDim.x) / kGroup;
     const int i = threadIdx.x;
     if (i < 3 * 256 / kGroup) {
         const int r = i / 3;
-        if (v0 + r < m.nv && s_ring[r]) grad[3 * v0 + i] = s_g[i];
+        if (v0 + r < m.nv && s_ring[r]) A.grad[3 * v0 + i] = s_g[i];
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pass2(PassArgs A)
+{
+    pass2_body<T>(A, logical_block());
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pass2_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
+                                                       int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    pass2_body<T>(A[r], lb - block0[r]);
 }
 
 // ------------------------------------------------------------ launchers ---
@@ -626,32 +713,78 @@ static inline int nblocks(int n, int b) { return (n + b - 1) / b; }
 
 int pass1_blocks(int nv) { return nblocks(nblocks(nv * kGroup, 256), kXcds) * kXcds; }
 
+PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
+                   double* grad)
+{
+    PassArgs a{};
+    a.m = m;
+    a.nl = nl;
+    a.p = p;
+    a.q = b.q;
+    a.gprops = b.gprops;
+    a.rep_v = (p.rep_lc1 > 0 && nl.kind >= 0) ? b.rep_v : nullptr;
+    a.pair_cnt = b.pair_cnt;
+    a.partials = b.partials;
+    a.globals = b.globals;
+    a.grad = grad;
+    a.vprops = vprops;
+    a.list = b.list;
+    a.nlist = b.nlist;
+    a.nblocks = pass1_blocks(m.nv);
+    return a;
+}
+
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
                   double* vprops, hipStream_t s)
 {
+    const PassArgs a = pass_args(m, nl, p, b, vprops, nullptr);
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
-    dim3 grid(pass1_blocks(m.nv)), block(256);
-    const bool rep_on = p.rep_lc1 > 0 && nl.kind >= 0;
-    if (rep_on) hipLaunchKernelGGL(k_rep, grid, block, 0, s, m, nl, p, b.rep_v, b.pair_cnt);
-    const double* rep_v = rep_on ? b.rep_v : nullptr;
-    if (mixed)
-        hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, rep_v, b.partials);
-    else
-        hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, m, nl, p, b.q, vprops, b.gprops, rep_v, b.partials);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, b.partials, (int)grid.x, p, b.globals);
+    dim3 grid(a.nblocks), block(256);
+    if (a.rep_v) hipLaunchKernelGGL(k_rep, grid, block, 0, s, a);
+    if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, a);
 }
 
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
                   double* grad, hipStream_t s)
 {
-    dim3 grid(pass1_blocks(m.nv)), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass2<float>, grid, block, 0, s, m, nl, p, b.q, b.globals, b.pair_cnt, grad);
-    else hipLaunchKernelGGL(k_pass2<double>, grid, block, 0, s, m, nl, p, b.q, b.globals, b.pair_cnt, grad);
+    const PassArgs a = pass_args(m, nl, p, b, nullptr, grad);
+    dim3 grid(a.nblocks), block(256);
+    if (mixed) hipLaunchKernelGGL(k_pass2<float>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_pass2<double>, grid, block, 0, s, a);
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_grad, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.q, b.globals,
                            b.list, b.nlist, grad);
+}
+
+void launch_pass1_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
+                          int total_blocks, int mixed, hipStream_t s)
+{
+    bool rep = false;
+    for (int r = 0; r < R; r++) {
+        if (h_args[r].nlist > 0)
+            hipLaunchKernelGGL(k_generic_props_at, dim3(nblocks(h_args[r].nlist, 128)), dim3(128), 0, s, d_args, r);
+        rep = rep || h_args[r].rep_v;
+    }
+    dim3 grid(total_blocks), block(256);
+    if (rep) hipLaunchKernelGGL(k_rep_grouped, grid, block, 0, s, d_args, d_block0, R);
+    if (mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, d_args, d_block0, R);
+    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, d_args, d_block0, R);
+    hipLaunchKernelGGL(k_finalize_grouped, dim3(R), dim3(1024), 0, s, d_args);
+}
+
+void launch_pass2_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
+                          int total_blocks, int mixed, hipStream_t s)
+{
+    dim3 grid(total_blocks), block(256);
+    if (mixed) hipLaunchKernelGGL(k_pass2_grouped<float>, grid, block, 0, s, d_args, d_block0, R);
+    else hipLaunchKernelGGL(k_pass2_grouped<double>, grid, block, 0, s, d_args, d_block0, R);
+    for (int r = 0; r < R; r++)
+        if (h_args[r].nlist > 0)
+            hipLaunchKernelGGL(k_generic_grad_at, dim3(nblocks(h_args[r].nlist, 128)), dim3(128), 0, s, d_args, r);
 }
 
 // --------------------------------------------- single-halfedge geometry ---

@@ -18,6 +18,36 @@ struct PassBuffers {
     int nlist;
 };
 int pass1_blocks(int nv);
+
+// Everything the pass kernels read and write for one mesh.  A grouped launch
+// takes a device array of these, one per replica, and a prefix table of
+// pass-1 blocks: logical block b belongs to the replica r with
+// block0[r] <= b < block0[r + 1] and is its local block b - block0[r], so
+// every replica is partitioned and summed exactly as in a launch of its own.
+struct PassArgs {
+    MeshView m;
+    NListView nl;
+    EParams p;
+    double* q;
+    double* gprops;
+    double* rep_v;       // null: repulsion off
+    int* pair_cnt;
+    double* partials;
+    Globals* globals;
+    double* grad;
+    double* vprops;      // optional per-vertex properties of pass 1
+    const int* list;     // boundary / degenerate vertices
+    int nlist;
+    int nblocks;         // pass-1 blocks (a multiple of 8)
+};
+PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
+                   double* grad);
+// R replicas; d_args, d_block0 in device memory, h_args the host copy (for
+// the launches of the boundary kernels)
+void launch_pass1_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
+                          int total_blocks, int mixed, hipStream_t s);
+void launch_pass2_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
+                          int total_blocks, int mixed, hipStream_t s);
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
                   double* vprops, hipStream_t s);
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
@@ -87,16 +117,49 @@ struct HmcState {
     double energy0, energy1, kinetic0, kinetic1, dh, uniform;
     int accepted, pad;
     unsigned long long n_accepted, n_steps;
+    double props0[6];    // global properties at x0 (first force evaluation)
+    double props[6];     // ... of the state after the step (x0 or xn)
 };
 enum { kHmcFirst = 1, kHmcKick = 2, kHmcDrift = 4, kHmcLast = 8 };
 int hmc_blocks(int n3);
 void launch_hmc_begin(int n3, const double* x, double* x0, double* p, const int* perm, int draw, uint64_t seed,
                       uint64_t step, double sigma, double* kpart, hipStream_t s);
 void launch_hmc_leap(int n3, double* x, double* p, double* a, const double* g, double dt, double dt2, double mass,
-                     int mode, const double* energy, HmcState* st, double* kpart, hipStream_t s);
+                     int mode, const Globals* globals, HmcState* st, double* kpart, hipStream_t s);
 void launch_hmc_accept(int n3, const double* kpart0, const double* kpart1, const double* x0, const double* x,
-                       int harmonic, const double* energy, double mass, double temperature, double uniform,
+                       int harmonic, const Globals* globals, double mass, double temperature, double uniform,
                        uint64_t seed, uint64_t step, HmcState* st, hipStream_t s);
 void launch_hmc_restore(int n3, double* x, const double* x0, const HmcState* st, hipStream_t s);
+
+// One replica of a grouped HMC step: device draws keyed by (seed, the
+// replica's own step count), forces from grad, energies from globals.
+struct HmcArgs {
+    int n3;
+    int nblocks;          // hmc_blocks(n3)
+    double* x;
+    double* x0;
+    double* p;
+    double* a;
+    const double* grad;
+    Globals* globals;
+    const int* perm;
+    HmcState* st;
+    double* kpart0;
+    double* kpart1;
+    double mass, dt, dt2, temperature;
+    uint64_t seed;
+};
+void launch_hmc_begin_grouped(const HmcArgs* d, const int* d_block0, int R, int total_blocks, hipStream_t s);
+void launch_hmc_leap_grouped(const HmcArgs* d, const int* d_block0, int R, int total_blocks, int mode,
+                             hipStream_t s);
+void launch_hmc_accept_grouped(const HmcArgs* d, int R, hipStream_t s);
+void launch_hmc_restore_grouped(const HmcArgs* d, const int* d_block0, int R, int total_blocks, hipStream_t s);
+// Parallel-tempering exchange over a kappa_b ladder: ladder slots s and s+1
+// (s = parity, parity + 2, ...) swap their replicas with probability
+// min(1, exp(-(k[s+1] - k[s]) (B_a - B_b) / T)), B the bending of the current
+// states; pass parameters and the slot owners are updated in place.
+void launch_pt_swap(PassArgs* d_args, const HmcArgs* d_hmc, int* owner, const double* ladder, int R,
+                    uint64_t seed, unsigned long long* round, unsigned long long* swaps_accepted,
+                    unsigned long long* swaps_tried, hipStream_t s);
 
 }    // namespace tmh
