@@ -117,6 +117,7 @@ int tm_energy_get_params(const tm_energy* em, tm_params* out);
 int tm_energy_initial_props(const tm_energy* em, tm_props* out);
 /* precision of properties/energy/gradient: TM_PREC_FP64 (default) or mixed */
 int tm_energy_set_precision(tm_energy* em, int precision);
+int tm_energy_get_precision(const tm_energy* em, int* precision);
 /* properties(mesh) -> reduced VertexProperties (energy.cpp:63-77) */
 int tm_energy_properties(tm_energy* em, tm_mesh* mesh, tm_props* out);
 /* per-vertex VertexProperties vector (EvaluateProperties, kernel.h:140-169) */
@@ -213,6 +214,23 @@ int tm_hmc_step(tm_hmc* hmc, const tm_hmc_params* params, double temperature, co
 /* result of the last step and running counts (synchronises) */
 int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
 
+/* ---- launch shapes and the autotuner ----
+ * No counterpart in the reference (OpenMP schedules).  The repulsion query
+ * and the gradient pass write per-vertex values only, so their block size and
+ * the number of candidates each lane keeps in flight change no result bit;
+ * the autotuner times every shape on the caller's mesh (HIP events, the mean
+ * of 5 launches after 2 untimed ones) and keeps the fastest. */
+typedef struct tm_tuning {
+    int rep_threads;     /* repulsion query: 64, 128, 256 or 512 threads per block */
+    int rep_batch;       /* candidates in flight per lane: 1, 2, 4 or 8 */
+    int pass2_threads;   /* gradient pass: 64, 128, 256 or 512 */
+    int reserved;
+} tm_tuning;
+int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t);
+int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out);
+/* ms_rep / ms_pass2 (may be NULL): the winners' times per launch */
+int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_rep, double* ms_pass2);
+
 /* ---- replica groups: batched meshes, parallel tempering, graph replay ----
  * No counterpart in the reference, which evaluates one mesh per call
  * (kernel.h:140-262).  R meshes with their energy managers, of any sizes,
@@ -258,6 +276,9 @@ int tm_replicas_stats(tm_replicas* rs, long long* accepted, long long* steps, in
 /* rebuild every replica's neighbour list from its current device positions
  * (EnergyManager.update_repulsion, energy.cpp:54-61) */
 int tm_replicas_update_repulsion(tm_replicas* rs);
+/* launch shapes of the grouped launches (see tm_energy_autotune) */
+int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t);
+int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double* ms_pass2);
 
 #ifdef __cplusplus
 }

@@ -170,3 +170,41 @@ def test_exchange_rule(core):
     assert decided == 9
     assert st2["swaps_tried"].sum() == 9
     assert 0 < st2["swaps_accepted"].sum() < 9    # both outcomes exercised
+
+
+SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64),
+          dict(rep_threads=512, rep_batch=8, pass2_threads=512),
+          dict(rep_threads=128, rep_batch=2, pass2_threads=128)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_launch_shapes_change_no_bits(core, precision):
+    """Every repulsion / pass-2 launch shape, and the autotuner's choice, gives
+    the default shape's energy and gradient bits (exclusion 0 so that every
+    vertex has repulsion pairs)."""
+    mesh, est = membrane(core, 4, 31, precision, excl=0)
+    e0, g0 = est.energy(mesh), est.gradient(mesh)
+    for sh in SHAPES:
+        est.tuning = sh
+        assert est.tuning == sh
+        assert est.energy(mesh) == e0 and np.array_equal(est.gradient(mesh), g0)
+    best = est.autotune(mesh)
+    print(precision, "autotuned", best, flush=True)
+    assert best["rep_threads"] in (64, 128, 256, 512) and best["rep_batch"] in (1, 2, 4, 8)
+    assert est.energy(mesh) == e0 and np.array_equal(est.gradient(mesh), g0)
+
+
+def test_grouped_shapes_and_autotune(core):
+    reps = mixed_set(core, "mixed")
+    R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+    R.eval()
+    ref = R.globals().copy(), [R.gradient(i) for i in range(len(reps))]
+    for sh in SHAPES:
+        R.tuning = sh
+        R.eval()
+        assert np.array_equal(R.globals(), ref[0])
+        assert all(np.array_equal(R.gradient(i), ref[1][i]) for i in range(len(reps)))
+    best = R.autotune()
+    print("group autotuned", best, flush=True)
+    R.eval()
+    assert np.array_equal(R.globals(), ref[0])

@@ -387,10 +387,29 @@ struct DeviceHMC {
     }
 };
 
+py::dict tuning_dict(const tm_tuning& t)
+{
+    py::dict d;
+    d["rep_threads"] = t.rep_threads;
+    d["rep_batch"] = t.rep_batch;
+    d["pass2_threads"] = t.pass2_threads;
+    return d;
+}
+
+tm_tuning tuning_of(const py::dict& d)
+{
+    tm_tuning t{};
+    t.rep_threads = d["rep_threads"].cast<int>();
+    t.rep_batch = d["rep_batch"].cast<int>();
+    t.pass2_threads = d["pass2_threads"].cast<int>();
+    return t;
+}
+
 // R meshes with their energy managers advanced together (tm_replicas_*).
 struct Replicas {
     tm_replicas* h = nullptr;
     std::vector<int> nv;
+    tm_tuning tune{256, 4, 256, 0};    // the library's default shape
     Replicas(const std::vector<TriMesh*>& meshes, const std::vector<EnergyManager*>& estores)
     {
         if (meshes.size() != estores.size()) throw std::runtime_error("one energy manager per mesh");
@@ -581,12 +600,39 @@ PYBIND11_MODULE(core, m)
         .def_property_readonly("initial_props", &EnergyManager::initial_props)
         .def_property(
             "precision",
-            [](EnergyManager&) -> std::string { return "unknown"; },
+            [](EnergyManager& e) -> std::string {
+                int p = 0;
+                check(tm_energy_get_precision(e.h, &p));
+                return p == TM_PREC_MIXED ? "mixed" : "fp64";
+            },
             [](EnergyManager& e, const std::string& s) {
                 if (s == "fp64") check(tm_energy_set_precision(e.h, TM_PREC_FP64));
                 else if (s == "mixed") check(tm_energy_set_precision(e.h, TM_PREC_MIXED));
                 else throw py::value_error("precision must be 'fp64' or 'mixed'");
-            });
+            })
+        .def_property(
+            "tuning",
+            [](EnergyManager& e) {
+                tm_tuning t;
+                check(tm_energy_get_tuning(e.h, &t));
+                return tuning_dict(t);
+            },
+            [](EnergyManager& e, const py::dict& d) {
+                tm_tuning t = tuning_of(d);
+                check(tm_energy_set_tuning(e.h, &t));
+            })
+        .def(
+            "autotune",
+            [](EnergyManager& e, TriMesh& t) {
+                tm_tuning out;
+                double ms_rep = 0, ms_p2 = 0;
+                check(tm_energy_autotune(e.h, t.h, &out, &ms_rep, &ms_p2));
+                py::dict d = tuning_dict(out);
+                d["rep_ms"] = ms_rep;
+                d["pass2_ms"] = ms_p2;
+                return d;
+            },
+            py::arg("mesh"));
 
     m.def(
         "flip",
@@ -714,7 +760,24 @@ PYBIND11_MODULE(core, m)
                  d["swaps_tried"] = stried;
                  return d;
              })
-        .def("update_repulsion", [](Replicas& r) { check(tm_replicas_update_repulsion(r.h)); });
+        .def("update_repulsion", [](Replicas& r) { check(tm_replicas_update_repulsion(r.h)); })
+        .def_property(
+            "tuning", [](Replicas& r) { return tuning_dict(r.tune); },
+            [](Replicas& r, const py::dict& d) {
+                tm_tuning t = tuning_of(d);
+                check(tm_replicas_set_tuning(r.h, &t));
+                r.tune = t;
+            })
+        .def("autotune", [](Replicas& r) {
+            tm_tuning out;
+            double ms_rep = 0, ms_p2 = 0;
+            check(tm_replicas_autotune(r.h, &out, &ms_rep, &ms_p2));
+            r.tune = out;
+            py::dict d = tuning_dict(out);
+            d["rep_ms"] = ms_rep;
+            d["pass2_ms"] = ms_p2;
+            return d;
+        });
 
     m.def(
         "make_nlist",

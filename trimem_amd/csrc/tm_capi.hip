@@ -541,6 +541,7 @@ struct tm_energy {
     DevBuf<int> flip_result, proposals, pf_status;
     DevBuf<double> uniforms, pf_dprops, pf_state;
     DevBuf<unsigned long long> pf_owner;
+    Tuning tune;
     ~tm_energy() { delete nlist; }
 };
 
@@ -603,14 +604,14 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
         HIPCHECK(hipMemsetAsync(E.globals.p, 0, sizeof(Globals), g_rt.stream));
         return;
     }
-    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), vprops, g_rt.stream);
+    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), vprops, g_rt.stream, E.tune);
     check_launch();
 }
 
 void run_pass2(tm_energy& E, tm_mesh& M, int mixed)
 {
     if (M.nv == 0) return;
-    launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), E.grad.p, g_rt.stream);
+    launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), E.grad.p, g_rt.stream, E.tune);
     check_launch();
 }
 
@@ -867,6 +868,12 @@ int tm_energy_set_precision(tm_energy* em, int precision)
     require(precision == TM_PREC_FP64 || precision == TM_PREC_MIXED, "unknown precision");
     em->precision = precision;
     API_END
+}
+
+int tm_energy_get_precision(const tm_energy* em, int* precision)
+{
+    *precision = em->precision;
+    return TM_OK;
 }
 
 int tm_energy_properties(tm_energy* em, tm_mesh* mesh, tm_props* out)
@@ -1284,6 +1291,14 @@ struct tm_replicas {
     std::vector<int> h_block0;
     DevBuf<int> d_block0;
     int total_blocks = 0;
+    Tuning tune;
+    DevBuf<int> d_block0_rep, d_block0_p2;
+    int total_rep = 0, total_p2 = 0;
+    GroupLaunch group() const
+    {
+        return GroupLaunch{d_args.p, h_args.data(), R, d_block0.p, total_blocks, d_block0_rep.p, total_rep,
+                           d_block0_p2.p, total_p2, tune, mixed};
+    }
     // HMC
     bool hmc_ready = false;
     int L = 0;
@@ -1316,6 +1331,8 @@ struct tm_replicas {
 
 namespace {
 
+void set_group_shape(tm_replicas& S, const Tuning& t);
+
 // Rebuild the device table of pass arguments from the replicas' current
 // buffers and neighbour lists.  With a ladder, each replica's kappa_b is that
 // of the slot it holds on the device.
@@ -1345,6 +1362,24 @@ void refresh_replica_args(tm_replicas& S)
     S.d_block0.alloc(S.R + 1);
     h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
     h2d(S.d_block0.p, S.h_block0.data(), sizeof(int) * (S.R + 1));
+    set_group_shape(S, S.tune);
+}
+
+// block prefix tables of the repulsion query and pass 2 for tuning t
+void set_group_shape(tm_replicas& S, const Tuning& t)
+{
+    std::vector<int> rep(S.R + 1, 0), p2(S.R + 1, 0);
+    for (int r = 0; r < S.R; r++) {
+        rep[r + 1] = rep[r] + tuned_blocks(S.meshes[r]->nv, t.rep_threads);
+        p2[r + 1] = p2[r] + tuned_blocks(S.meshes[r]->nv, t.p2_threads);
+    }
+    S.tune = t;
+    S.total_rep = rep[S.R];
+    S.total_p2 = p2[S.R];
+    S.d_block0_rep.alloc(S.R + 1);
+    S.d_block0_p2.alloc(S.R + 1);
+    h2d(S.d_block0_rep.p, rep.data(), sizeof(int) * (S.R + 1));
+    h2d(S.d_block0_p2.p, p2.data(), sizeof(int) * (S.R + 1));
     sync();
 }
 
@@ -1359,8 +1394,8 @@ void replicas_hmc_step(tm_replicas& S)
 {
     launch_hmc_begin_grouped(S.d_hmc.p, S.d_hblock0.p, S.R, S.hmc_total, g_rt.stream);
     for (int i = 0; i <= S.L; i++) {
-        launch_pass1_grouped(S.d_args.p, S.d_block0.p, S.h_args.data(), S.R, S.total_blocks, S.mixed, g_rt.stream);
-        launch_pass2_grouped(S.d_args.p, S.d_block0.p, S.h_args.data(), S.R, S.total_blocks, S.mixed, g_rt.stream);
+        launch_pass1_grouped(S.group(), g_rt.stream);
+        launch_pass2_grouped(S.group(), g_rt.stream);
         int mode = i == 0 ? kHmcFirst : kHmcKick;
         mode |= i < S.L ? kHmcDrift : kHmcLast;
         launch_hmc_leap_grouped(S.d_hmc.p, S.d_hblock0.p, S.R, S.hmc_total, mode, g_rt.stream);
@@ -1402,12 +1437,8 @@ int tm_replicas_eval(tm_replicas* rs, int flags)
 {
     API_BEGIN
     replicas_device_points(*rs);
-    if (flags & TM_EVAL_PROPS)
-        launch_pass1_grouped(rs->d_args.p, rs->d_block0.p, rs->h_args.data(), rs->R, rs->total_blocks, rs->mixed,
-                             g_rt.stream);
-    if (flags & TM_EVAL_GRAD)
-        launch_pass2_grouped(rs->d_args.p, rs->d_block0.p, rs->h_args.data(), rs->R, rs->total_blocks, rs->mixed,
-                             g_rt.stream);
+    if (flags & TM_EVAL_PROPS) launch_pass1_grouped(rs->group(), g_rt.stream);
+    if (flags & TM_EVAL_GRAD) launch_pass2_grouped(rs->group(), g_rt.stream);
     check_launch();
     API_END
 }
@@ -1603,6 +1634,152 @@ int tm_replicas_update_repulsion(tm_replicas* rs)
         E.nlist = fresh;
     }
     refresh_replica_args(S);
+    API_END
+}
+
+}    // extern "C"
+
+// ---- launch-shape autotuner ----
+namespace {
+
+Tuning from_c(const tm_tuning& c)
+{
+    Tuning t;
+    t.rep_threads = c.rep_threads;
+    t.rep_batch = c.rep_batch;
+    t.p2_threads = c.pass2_threads;
+    return t;
+}
+
+tm_tuning to_c(const Tuning& t)
+{
+    tm_tuning c{};
+    c.rep_threads = t.rep_threads;
+    c.rep_batch = t.rep_batch;
+    c.pass2_threads = t.p2_threads;
+    return c;
+}
+
+// mean time of `reps` launches of f after two untimed ones (ms)
+template <class F>
+double time_launches(F&& f, int reps)
+{
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    f();
+    f();
+    HIPCHECK(hipEventRecord(a, g_rt.stream));
+    for (int i = 0; i < reps; i++) f();
+    HIPCHECK(hipEventRecord(b, g_rt.stream));
+    check_launch();
+    HIPCHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
+// Try every shape of the repulsion query and of pass 2 with launch(t) and
+// keep the fastest of each; the two kernels are timed separately.
+template <class RepF, class P2F>
+Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, double* best_rep_ms, double* best_p2_ms)
+{
+    const int ths[4] = {64, 128, 256, 512}, bats[4] = {1, 2, 4, 8};
+    Tuning best = base;
+    double brep = 1e30, bp2 = 1e30;
+    if (has_rep)
+        for (int th : ths)
+            for (int ba : bats) {
+                Tuning t = best;
+                t.rep_threads = th;
+                t.rep_batch = ba;
+                const double ms = time_launches([&] { rep(t); }, 5);
+                if (ms < brep) {
+                    brep = ms;
+                    best.rep_threads = th;
+                    best.rep_batch = ba;
+                }
+            }
+    for (int th : ths) {
+        Tuning t = best;
+        t.p2_threads = th;
+        const double ms = time_launches([&] { p2(t); }, 5);
+        if (ms < bp2) {
+            bp2 = ms;
+            best.p2_threads = th;
+        }
+    }
+    if (best_rep_ms) *best_rep_ms = has_rep ? brep : 0.0;
+    if (best_p2_ms) *best_p2_ms = bp2;
+    return best;
+}
+
+}    // namespace
+
+extern "C" {
+
+int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t)
+{
+    API_BEGIN
+    require(t != nullptr && tuning_valid(from_c(*t)), "invalid launch shape");
+    em->tune = from_c(*t);
+    API_END
+}
+
+int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out)
+{
+    *out = to_c(em->tune);
+    return TM_OK;
+}
+
+int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_rep, double* ms_pass2)
+{
+    API_BEGIN
+    tm_energy& E = *em;
+    tm_mesh& M = *mesh;
+    upload_points(M);
+    run_pass1(E, M, E.precision, nullptr);    // pass 2 reads q and the globals
+    const PassArgs a = pass_args(M.view(), E.nlist->view(), eparams_of(E), buffers(E, M), nullptr, E.grad.p);
+    if (M.nv > 0)
+        E.tune = search(
+            E.tune, a.rep_v != nullptr, [&](const Tuning& t) { launch_rep(a, t, g_rt.stream); },
+            [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); }, ms_rep, ms_pass2);
+    if (out) *out = to_c(E.tune);
+    API_END
+}
+
+int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t)
+{
+    API_BEGIN
+    require(t != nullptr && tuning_valid(from_c(*t)), "invalid launch shape");
+    rs->drop_graph();
+    set_group_shape(*rs, from_c(*t));
+    API_END
+}
+
+int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double* ms_pass2)
+{
+    API_BEGIN
+    tm_replicas& S = *rs;
+    replicas_device_points(S);
+    launch_pass1_grouped(S.group(), g_rt.stream);
+    bool has_rep = false;
+    for (const PassArgs& a : S.h_args) has_rep = has_rep || a.rep_v;
+    // a new shape's tables are uploaded in the untimed warm-up launches
+    auto shaped = [&](const Tuning& t) {
+        if (t.rep_threads != S.tune.rep_threads || t.rep_batch != S.tune.rep_batch ||
+            t.p2_threads != S.tune.p2_threads)
+            set_group_shape(S, t);
+        return S.group();
+    };
+    const Tuning best = search(
+        S.tune, has_rep, [&](const Tuning& t) { launch_rep_grouped(shaped(t), g_rt.stream); },
+        [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, ms_rep, ms_pass2);
+    S.drop_graph();
+    set_group_shape(S, best);
+    if (out) *out = to_c(best);
     API_END
 }
 

@@ -470,15 +470,15 @@ __global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restric
 // chain of dependent loads (cell, candidate id, position, ring of the
 // candidate) and needs few registers, so it runs apart from the
 // register-heavy ring work of pass 1.
-constexpr int kRepBatch = 4;
+template <int kThreads, int kBatch>
 __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
 {
     if (!A.rep_v) return;
     const MeshView& m = A.m;
     const EParams& p = A.p;
-    __shared__ double s_rep[256 / kGroup];
-    __shared__ int s_cnt[256 / kGroup];
-    const int t = lb * blockDim.x + threadIdx.x;
+    __shared__ double s_rep[kThreads / kGroup];
+    __shared__ int s_cnt[kThreads / kGroup];
+    const int t = lb * kThreads + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
@@ -487,7 +487,7 @@ __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
     int cnt = 0;
     if (ring) {
         const d3 xv = load_p(m.x, v);
-        group_neighbours<kRepBatch>(m, A.nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
+        group_neighbours<kBatch>(m, A.nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
             rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
             cnt++;
         });
@@ -499,22 +499,27 @@ __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
         s_cnt[threadIdx.x / kGroup] = cnt;
     }
     __syncthreads();
-    const int v0 = (lb * blockDim.x) / kGroup;
+    const int v0 = (lb * kThreads) / kGroup;
     const int i = threadIdx.x;
-    if (i < 256 / kGroup && v0 + i < m.nv) {
+    if (i < kThreads / kGroup && v0 + i < m.nv) {
         A.rep_v[v0 + i] = s_rep[i];
         A.pair_cnt[v0 + i] = s_cnt[i];
     }
 }
 
-__global__ void __launch_bounds__(256) k_rep(PassArgs A) { rep_body(A, logical_block()); }
+template <int kThreads, int kBatch>
+__global__ void __launch_bounds__(kThreads) k_rep(PassArgs A)
+{
+    rep_body<kThreads, kBatch>(A, logical_block());
+}
 
-__global__ void __launch_bounds__(256) k_rep_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
-                                                     int R)
+template <int kThreads, int kBatch>
+__global__ void __launch_bounds__(kThreads) k_rep_grouped(const PassArgs* __restrict__ A,
+                                                          const int* __restrict__ block0, int R)
 {
     const int lb = logical_block();
     const int r = find_replica(block0, R, lb);
-    rep_body(A[r], lb - block0[r]);
+    rep_body<kThreads, kBatch>(A[r], lb - block0[r]);
 }
 
 // Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
@@ -644,13 +649,13 @@ __device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p,
     return acc;
 }
 
-template <class T>
+template <class T, int kThreads>
 __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
     const NListView& nl = A.nl;
     const EParams& p = A.p;
-    const int t = lb * blockDim.x + threadIdx.x;
+    const int t = lb * kThreads + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
@@ -675,8 +680,8 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
     acc.z = group_sum(acc.z);
     // stage the block's 32 gradient rows and store them as contiguous doubles;
     // rows of listed (boundary) vertices are left to k_generic_grad
-    __shared__ double s_g[3 * 256 / kGroup];
-    __shared__ int s_ring[256 / kGroup];
+    __shared__ double s_g[3 * kThreads / kGroup];
+    __shared__ int s_ring[kThreads / kGroup];
     if (lane == 0) {
         const int r = threadIdx.x / kGroup;
         s_g[3 * r] = acc.x;
@@ -685,27 +690,27 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
         s_ring[r] = ring;
     }
     __syncthreads();
-    const int v0 = (lb * blockDim.x) / kGroup;
+    const int v0 = (lb * kThreads) / kGroup;
     const int i = threadIdx.x;
-    if (i < 3 * 256 / kGroup) {
+    if (i < 3 * kThreads / kGroup) {
         const int r = i / 3;
         if (v0 + r < m.nv && s_ring[r]) A.grad[3 * v0 + i] = s_g[i];
     }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_pass2(PassArgs A)
+template <class T, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_pass2(PassArgs A)
 {
-    pass2_body<T>(A, logical_block());
+    pass2_body<T, kThreads>(A, logical_block());
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_pass2_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
-                                                       int R)
+template <class T, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_pass2_grouped(const PassArgs* __restrict__ A,
+                                                            const int* __restrict__ block0, int R)
 {
     const int lb = logical_block();
     const int r = find_replica(block0, R, lb);
-    pass2_body<T>(A[r], lb - block0[r]);
+    pass2_body<T, kThreads>(A[r], lb - block0[r]);
 }
 
 // ------------------------------------------------------------ launchers ---
@@ -734,57 +739,129 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
     return a;
 }
 
+bool tuning_valid(const Tuning& t)
+{
+    auto th = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512; };
+    return th(t.rep_threads) && th(t.p2_threads) &&
+           (t.rep_batch == 1 || t.rep_batch == 2 || t.rep_batch == 4 || t.rep_batch == 8);
+}
+
+int tuned_blocks(int nv, int threads) { return nblocks(nblocks(nv * kGroup, threads), kXcds) * kXcds; }
+
+template <int TH, int B>
+static void rep_go(bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
+{
+    if (grouped)
+        hipLaunchKernelGGL((k_rep_grouped<TH, B>), dim3(g->total_rep), dim3(TH), 0, s, g->d_args, g->d_block0_rep,
+                           g->R);
+    else hipLaunchKernelGGL((k_rep<TH, B>), dim3(tuned_blocks(a.m.nv, TH)), dim3(TH), 0, s, a);
+}
+template <int TH>
+static void rep_batch(int batch, bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
+{
+    switch (batch) {
+    case 1: rep_go<TH, 1>(grouped, a, g, s); break;
+    case 2: rep_go<TH, 2>(grouped, a, g, s); break;
+    case 8: rep_go<TH, 8>(grouped, a, g, s); break;
+    default: rep_go<TH, 4>(grouped, a, g, s); break;
+    }
+}
+static void rep_dispatch(const Tuning& t, bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
+{
+    switch (t.rep_threads) {
+    case 64: rep_batch<64>(t.rep_batch, grouped, a, g, s); break;
+    case 128: rep_batch<128>(t.rep_batch, grouped, a, g, s); break;
+    case 512: rep_batch<512>(t.rep_batch, grouped, a, g, s); break;
+    default: rep_batch<256>(t.rep_batch, grouped, a, g, s); break;
+    }
+}
+
+template <class T, int TH>
+static void p2_go(bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
+{
+    if (grouped)
+        hipLaunchKernelGGL((k_pass2_grouped<T, TH>), dim3(g->total_p2), dim3(TH), 0, s, g->d_args, g->d_block0_p2,
+                           g->R);
+    else hipLaunchKernelGGL((k_pass2<T, TH>), dim3(tuned_blocks(a.m.nv, TH)), dim3(TH), 0, s, a);
+}
+template <class T>
+static void p2_threads(int th, bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
+{
+    switch (th) {
+    case 64: p2_go<T, 64>(grouped, a, g, s); break;
+    case 128: p2_go<T, 128>(grouped, a, g, s); break;
+    case 512: p2_go<T, 512>(grouped, a, g, s); break;
+    default: p2_go<T, 256>(grouped, a, g, s); break;
+    }
+}
+static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs& a, const GroupLaunch* g,
+                        hipStream_t s)
+{
+    if (mixed) p2_threads<float>(t.p2_threads, grouped, a, g, s);
+    else p2_threads<double>(t.p2_threads, grouped, a, g, s);
+}
+
+void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s)
+{
+    if (a.rep_v && a.m.nv > 0) rep_dispatch(t, false, a, nullptr, s);
+}
+
+void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
+{
+    if (a.m.nv > 0) p2_dispatch(mixed, t, false, a, nullptr, s);
+}
+
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* vprops, hipStream_t s)
+                  double* vprops, hipStream_t s, const Tuning& t)
 {
     const PassArgs a = pass_args(m, nl, p, b, vprops, nullptr);
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
+    launch_rep(a, t, s);
     dim3 grid(a.nblocks), block(256);
-    if (a.rep_v) hipLaunchKernelGGL(k_rep, grid, block, 0, s, a);
     if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
     else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, a);
 }
 
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* grad, hipStream_t s)
+                  double* grad, hipStream_t s, const Tuning& t)
 {
     const PassArgs a = pass_args(m, nl, p, b, nullptr, grad);
-    dim3 grid(a.nblocks), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass2<float>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_pass2<double>, grid, block, 0, s, a);
+    launch_pass2_kernel(a, mixed, t, s);
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_grad, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.q, b.globals,
                            b.list, b.nlist, grad);
 }
 
-void launch_pass1_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
-                          int total_blocks, int mixed, hipStream_t s)
+void launch_rep_grouped(const GroupLaunch& g, hipStream_t s)
 {
     bool rep = false;
-    for (int r = 0; r < R; r++) {
-        if (h_args[r].nlist > 0)
-            hipLaunchKernelGGL(k_generic_props_at, dim3(nblocks(h_args[r].nlist, 128)), dim3(128), 0, s, d_args, r);
-        rep = rep || h_args[r].rep_v;
-    }
-    dim3 grid(total_blocks), block(256);
-    if (rep) hipLaunchKernelGGL(k_rep_grouped, grid, block, 0, s, d_args, d_block0, R);
-    if (mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, d_args, d_block0, R);
-    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, d_args, d_block0, R);
-    hipLaunchKernelGGL(k_finalize_grouped, dim3(R), dim3(1024), 0, s, d_args);
+    for (int r = 0; r < g.R; r++) rep = rep || g.h_args[r].rep_v;
+    if (rep) rep_dispatch(g.tune, true, g.h_args[0], &g, s);
 }
 
-void launch_pass2_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
-                          int total_blocks, int mixed, hipStream_t s)
+void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
 {
-    dim3 grid(total_blocks), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass2_grouped<float>, grid, block, 0, s, d_args, d_block0, R);
-    else hipLaunchKernelGGL(k_pass2_grouped<double>, grid, block, 0, s, d_args, d_block0, R);
-    for (int r = 0; r < R; r++)
-        if (h_args[r].nlist > 0)
-            hipLaunchKernelGGL(k_generic_grad_at, dim3(nblocks(h_args[r].nlist, 128)), dim3(128), 0, s, d_args, r);
+    for (int r = 0; r < g.R; r++)
+        if (g.h_args[r].nlist > 0)
+            hipLaunchKernelGGL(k_generic_props_at, dim3(nblocks(g.h_args[r].nlist, 128)), dim3(128), 0, s, g.d_args,
+                               r);
+    launch_rep_grouped(g, s);
+    dim3 grid(g.total_p1), block(256);
+    if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(1024), 0, s, g.d_args);
+}
+
+void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s)
+{
+    p2_dispatch(g.mixed, g.tune, true, g.h_args[0], &g, s);
+    for (int r = 0; r < g.R; r++)
+        if (g.h_args[r].nlist > 0)
+            hipLaunchKernelGGL(k_generic_grad_at, dim3(nblocks(g.h_args[r].nlist, 128)), dim3(128), 0, s, g.d_args,
+                               r);
 }
 
 // --------------------------------------------- single-halfedge geometry ---

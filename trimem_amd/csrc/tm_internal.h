@@ -42,16 +42,43 @@ struct PassArgs {
 };
 PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
                    double* grad);
-// R replicas; d_args, d_block0 in device memory, h_args the host copy (for
-// the launches of the boundary kernels)
-void launch_pass1_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
-                          int total_blocks, int mixed, hipStream_t s);
-void launch_pass2_grouped(const PassArgs* d_args, const int* d_block0, const PassArgs* h_args, int R,
-                          int total_blocks, int mixed, hipStream_t s);
+// Launch shapes that do not change any result bit: the repulsion query and
+// pass 2 write per-vertex values only, and each lane visits its candidates in
+// the same order whatever the batch.  (Pass 1's blocks fix the order of the
+// global sums and stay at 256 threads.)
+struct Tuning {
+    int rep_threads = 256;    // 64, 128, 256 or 512
+    int rep_batch = 4;        // candidates in flight per lane: 1, 2, 4 or 8
+    int p2_threads = 256;     // 64, 128, 256 or 512
+};
+bool tuning_valid(const Tuning& t);
+int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
+// the repulsion query and the pass-2 kernel alone (for timing)
+void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s);
+void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s);
+// R replicas: d_args in device memory, h_args its host copy (for the
+// launches of the boundary kernels), one prefix table of blocks per kernel
+// shape
+struct GroupLaunch {
+    const PassArgs* d_args;
+    const PassArgs* h_args;
+    int R;
+    const int* d_block0_p1;
+    int total_p1;
+    const int* d_block0_rep;
+    int total_rep;
+    const int* d_block0_p2;
+    int total_p2;
+    Tuning tune;
+    int mixed;
+};
+void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s);
+void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s);
+void launch_rep_grouped(const GroupLaunch& g, hipStream_t s);
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* vprops, hipStream_t s);
+                  double* vprops, hipStream_t s, const Tuning& t = Tuning());
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* grad, hipStream_t s);
+                  double* grad, hipStream_t s, const Tuning& t = Tuning());
 void launch_halfedge_geometry(const MeshView& m, int h, double* out, hipStream_t s);
 void launch_sum_geometry(const MeshView& m, int what, double* partials, unsigned* counter, double* result,
                          hipStream_t s);
