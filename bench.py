@@ -179,6 +179,93 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
                           "neighbour list rebuilt every step"}
 
 
+def replica_workload(n_replicas=64, seed=20261204):
+    """The batched-replica configuration: meshes of icosphere levels drawn
+    uniformly from 3-6 (642 to 40962 vertices), volume fractions U[0.6, 0.95],
+    kappa_b log-uniform on [10, 100] sorted into a tempering ladder; other
+    parameters as workload()."""
+    from trimem_amd import meshes
+    rng = np.random.default_rng(seed)
+    levels = rng.integers(3, 7, size=n_replicas)
+    vfrac = rng.uniform(0.6, 0.95, size=n_replicas)
+    ladder = np.sort(np.exp(rng.uniform(np.log(10.0), np.log(100.0), size=n_replicas)))
+    out = []
+    for r in range(n_replicas):
+        v, f, lbar = meshes.jittered_icosphere(int(levels[r]), seed=seed + 1 + r, rel_sigma=0.01)
+        params = dict(kappa_b=float(ladder[r]), kappa_a=1e6, kappa_v=1e6, kappa_c=0.0, kappa_t=1e5, kappa_r=1e3,
+                      area_frac=1.0, volume_frac=float(vfrac[r]), curvature_frac=1.0,
+                      bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, bond_a0=1.0, bond_r=2, bond_type=0,
+                      rep_lc1=1.5 * lbar, rep_r=2, rep_nlist=0, rep_rlist=1.5 * lbar, rep_exclusion=2,
+                      cont_delta=0.0, cont_lambda=1.0)
+        out.append((v, f, params))
+    return out, ladder
+
+
+def replica_rates(precision, evals, hmc_steps, hip, stream):
+    """Batched-replica rates: one grouped evaluation of all replicas (every
+    kernel one launch over all meshes) against the same meshes evaluated one
+    by one, and grouped HMC steps (dt 1e-5, 10 leapfrog steps, device draws,
+    replica exchange every 10 steps) with and without hipGraph replay."""
+    from trimem_amd import capi, core
+    L = capi.lib()
+    work, ladder = replica_workload()
+    meshes, ests = [], []
+    for v, f, params in work:
+        m = core.TriMesh(v, f)
+        e = core.EnergyManager(m, core_params(core, params))
+        e.precision = precision
+        meshes.append(m)
+        ests.append(e)
+    R = core.Replicas(meshes, ests)
+    tune = R.autotune()
+    nv = np.array([m.n_vertices() for m in meshes])
+    nf = np.array([m.n_faces() for m in meshes])
+    eval_bytes = float(np.sum((24.0 * nv + 12.0 * nf + 16.0 * nv) * 2 + 24.0 * nv))
+
+    def timed(fn, k):
+        fn()
+        capi.check(L.tm_sync())
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        for _ in range(k):
+            fn()
+        hip.record(e1, stream)
+        return hip.elapsed_ms(e0, e1) / k
+
+    ms_group = timed(lambda: R.eval(), evals)
+    handles = [(ctypes.c_void_p(capi_handle(m)), ctypes.c_void_p(capi_handle(e))) for m, e in zip(meshes, ests)]
+    both = capi.TM_EVAL_PROPS | capi.TM_EVAL_GRAD
+
+    def one_by_one():
+        for mh, eh in handles:
+            L.tm_energy_eval_device(eh, mh, both)
+    ms_seq = timed(one_by_one, max(2, evals // 4))
+
+    R.hmc_setup(mass=1.0, time_step=1e-5, temperature=1.0, seeds=list(range(1, len(meshes) + 1)),
+                num_integration_steps=10)
+    R.set_ladder(list(ladder), seed=7)
+    ms_nograph = timed(lambda: R.hmc_run(10, swap_every=10, use_graph=False), max(1, hmc_steps // 10)) / 10
+    ms_graph = timed(lambda: R.hmc_run(10, swap_every=10, use_graph=True), max(1, hmc_steps // 10)) / 10
+    st = R.stats()
+    return {"replicas": len(meshes), "vertices": int(nv.sum()), "levels": sorted(set(int(x) for x in
+                                                                                   np.round(np.log(
+                                                                                       (nv - 2) / 10) / np.log(4)))),
+            "group_evals_per_s": 1e3 / ms_group, "replica_evals_per_s": len(meshes) * 1e3 / ms_group,
+            "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9, "algorithmic_bytes_per_group_eval": eval_bytes,
+            "one_by_one_group_evals_per_s": 1e3 / ms_seq, "grouped_over_one_by_one": ms_seq / ms_group,
+            "hmc_group_steps_per_s_graph": 1e3 / ms_graph, "hmc_group_steps_per_s_direct": 1e3 / ms_nograph,
+            "hmc_acceptance": float(st["accepted"].sum() / max(st["steps"].sum(), 1)),
+            "swap_acceptance": float(st["swaps_accepted"].sum() / max(st["swaps_tried"].sum(), 1)),
+            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads")},
+            "workload": "64 icosphere meshes, levels 3-6 from seed, vfrac U[0.6,0.95], kappa_b log-uniform "
+                        "[10,100] ladder; HMC dt 1e-5, 10 leapfrog steps, exchange every 10 steps"}
+
+
+def capi_handle(obj):
+    """The C handle behind a trimem_amd.core TriMesh / EnergyManager."""
+    return obj._handle()
+
+
 def _csv_rows(path):
     import csv
     return list(csv.DictReader(open(path)))
@@ -294,6 +381,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--replica-evals", type=int, default=40,
+                    help="grouped evaluations of the 64-replica set timed for the replicas block (0: skip)")
     ap.add_argument("--hmc-steps", type=int, default=40,
                     help="device-resident HMC steps timed for the hmc block (0: skip)")
     ap.add_argument("--summarize", nargs="+", metavar="DIR",
@@ -302,14 +391,14 @@ def main():
     args = ap.parse_args()
     if args.summarize:
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
-               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0",
+               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0 --replica-evals 0",
                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + args.summarize[1] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0",
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0",
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + args.summarize[2] +
                " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline"]
         if len(args.summarize) > 3:
             cmd.append("rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d " +
-                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0")
+                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0")
         cmd.append("python3 bench.py --summarize " + " ".join(args.summarize))
         summarize(args.summarize[0], args.summarize[1], args.summarize[2], PROFILE_JSON,
                   os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"), cmd,
@@ -427,6 +516,8 @@ def main():
     }
     if args.hmc_steps > 0:
         out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))
+    if args.replica_evals > 0:
+        out["replicas"] = replica_rates(args.precision, args.replica_evals, args.hmc_steps, hip, stream)
     if not args.no_cpu_baseline:
         cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",

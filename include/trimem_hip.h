@@ -237,7 +237,10 @@ int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_
  * advance together: every kernel of an evaluation or of an HMC step is one
  * launch over all replicas, and each replica's results equal, bit for bit,
  * those of the single-mesh calls (tm_energy_eval_device, tm_hmc_step with
- * device draws).  All energy managers must use the same precision. */
+ * device draws).  All energy managers must use the same precision.  Like
+ * tm_energy_eval_device, the group works on the device positions: creation
+ * uploads the host arrays that are newer, later host writes need
+ * tm_mesh_upload_points. */
 typedef struct tm_replicas tm_replicas;
 int tm_replicas_create(tm_mesh* const* meshes, tm_energy* const* ems, int n_replicas, tm_replicas** out);
 int tm_replicas_destroy(tm_replicas* rs);

@@ -465,6 +465,7 @@ PYBIND11_MODULE(core, m)
         .def("n_vertices", [](TriMesh& t) { return t.nv(); })
         .def("n_edges", [](TriMesh& t) { return t.ne(); })
         .def("n_faces", [](TriMesh& t) { return t.nf(); })
+        .def("_handle", [](TriMesh& t) { return (uintptr_t)t.h; }, "the tm_mesh* of the C ABI")
         .def("halfedge_handle", [](TriMesh& t, int i) { return HalfedgeHandle(i >= 0 && i < 2 * t.ne() ? i : -1); },
              py::arg("edge"))
         .def("opposite_halfedge_handle",
@@ -563,6 +564,7 @@ PYBIND11_MODULE(core, m)
 
     py::class_<EnergyManager>(m, "EnergyManager")
         .def(py::init<TriMesh&, const EnergyParams&>(), py::arg("mesh"), py::arg("eparams"))
+        .def("_handle", [](EnergyManager& e) { return (uintptr_t)e.h; }, "the tm_energy* of the C ABI")
         .def("properties", &EnergyManager::properties, py::arg("mesh"))
         .def("energy", &EnergyManager::energy, py::arg("mesh"))
         .def("energy", &EnergyManager::energy_props, py::arg("vprops"))

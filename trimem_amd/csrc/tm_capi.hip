@@ -1423,6 +1423,7 @@ int tm_replicas_create(tm_mesh* const* meshes, tm_energy* const* ems, int n_repl
     }
     S->mixed = ems[0]->precision;
     refresh_replica_args(*S);
+    replicas_device_points(*S);
     *out = S.release();
     API_END
 }
@@ -1436,7 +1437,6 @@ int tm_replicas_destroy(tm_replicas* rs)
 int tm_replicas_eval(tm_replicas* rs, int flags)
 {
     API_BEGIN
-    replicas_device_points(*rs);
     if (flags & TM_EVAL_PROPS) launch_pass1_grouped(rs->group(), g_rt.stream);
     if (flags & TM_EVAL_GRAD) launch_pass2_grouped(rs->group(), g_rt.stream);
     check_launch();
@@ -1570,7 +1570,6 @@ int tm_replicas_hmc_run(tm_replicas* rs, int n_steps, int swap_every, int use_gr
     require(S.hmc_ready, "tm_replicas_hmc_setup first");
     require(n_steps >= 0, "n_steps must be >= 0");
     require(swap_every <= 0 || S.has_ladder, "exchange needs a ladder");
-    replicas_device_points(S);
     if (use_graph && !S.graph_exec && n_steps > 0) {
         // the step reads every changing quantity (step counts, kappa_b, list
         // pointers) from device tables, so one capture serves all steps
@@ -1763,7 +1762,6 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
 {
     API_BEGIN
     tm_replicas& S = *rs;
-    replicas_device_points(S);
     launch_pass1_grouped(S.group(), g_rt.stream);
     bool has_rep = false;
     for (const PassArgs& a : S.h_args) has_rep = has_rep || a.rep_v;
