@@ -283,6 +283,42 @@ int tm_replicas_update_repulsion(tm_replicas* rs);
 int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t);
 int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double* ms_pass2);
 
+/* ---- surrogate forces (no counterpart in the reference) ----
+ * A one-ring message-passing network (width 64, three layers, bf16 MFMA
+ * products with FP32 accumulation) that predicts F = -dE/dx from invariant
+ * ring features; trained on the GPU by its own forward and backward kernels
+ * from (positions, exact gradient) snapshots.  HMC steps driven by it are
+ * accepted with the exact energy, so sampling stays exact.  Closed meshes
+ * with valences 3..10. */
+typedef struct tm_surrogate tm_surrogate;
+/* random initial weights from seed; feature normalisation and force scale
+ * from the mesh's current state */
+int tm_surrogate_create(tm_mesh* mesh, tm_energy* em, uint64_t seed, tm_surrogate** out);
+int tm_surrogate_destroy(tm_surrogate* sg);
+/* the FP32 parameter vector (out may be NULL to query n) */
+int tm_surrogate_params(tm_surrogate* sg, float* out, int* n);
+int tm_surrogate_set_params(tm_surrogate* sg, const float* in);
+/* feature means and scales (8 each), mean edge length, force scale */
+int tm_surrogate_norm(tm_surrogate* sg, double* mu, double* sig, double* lbar, double* sigma_F);
+/* predicted forces at the current positions, double[n*3] user order (may be NULL) */
+int tm_surrogate_forces(tm_surrogate* sg, double* forces);
+/* loss against the exact gradient at the current positions and its
+ * gradient with respect to the parameters (grad may be NULL) */
+int tm_surrogate_loss_grad(tm_surrogate* sg, double* loss, float* grad);
+/* activations of the last forward pass, user vertex order: which 0 the
+ * input features (n x 16), 1..4 the hidden states H_0..H_3, 5..7 the
+ * aggregated messages M_1..M_3 (n x 64) */
+int tm_surrogate_activations(tm_surrogate* sg, int which, float* out);
+/* record the current positions and their exact gradient on the device */
+int tm_surrogate_add_snapshot(tm_surrogate* sg);
+int tm_surrogate_num_snapshots(tm_surrogate* sg, int* n);
+/* Adam (lr) over the snapshots, reshuffled (seed) every epoch; the mesh
+ * positions are restored afterwards; mean_loss: mean loss of the last epoch */
+int tm_surrogate_train(tm_surrogate* sg, int epochs, float lr, uint64_t seed, double* mean_loss);
+/* tm_hmc_step with the surrogate's forces and the exact energies */
+int tm_hmc_step_surrogate(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* params, double temperature,
+                          const double* momenta, double uniform, uint64_t seed, uint64_t step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,183 @@
+"""GPU: the surrogate force network (trimem_amd.core.Surrogate, tm_surrogate_*).
+
+Pinned by:
+  * forward: the GPU's predicted forces against the CPU restatement
+    (tests/surrogate_ref.py, bf16-rounded matrix operands like the MFMA
+    kernels, features rebuilt from the oracle's properties), relative error
+    <= 1e-2 over the whole force array;
+  * backward: the GPU parameter gradient of the loss against torch autograd
+    of the same restatement: cosine >= 0.99 for every weight tensor, relative
+    error <= 5e-2 overall (the restatement itself is pinned by finite
+    differences in tests/test_surrogate_ref.py);
+  * training on GPU-recorded snapshots lowers the loss, on the snapshots and
+    on a held-out state;
+  * exact acceptance: a surrogate-driven HMC step reports the exact energies
+    of its two end points (EnergyManager.energy at those positions).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from trimem_amd import meshes as M
+import surrogate_ref as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def core():
+    import trimem_amd.core as m
+    assert m.device_count() > 0, "no GPU visible"
+    return m
+
+
+def pdict(lbar):
+    return dict(kappa_b=1.0, kappa_a=1e3, kappa_v=1e3, kappa_c=1.0, kappa_t=10.0, kappa_r=1.0, volume_frac=0.9,
+                bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, rep_lc1=1.2 * lbar, rep_rlist=1.2 * lbar,
+                rep_exclusion=0)
+
+
+def core_params(m, d):
+    e = m.EnergyParams()
+    for k in ("kappa_b", "kappa_a", "kappa_v", "kappa_c", "kappa_t", "kappa_r", "volume_frac"):
+        setattr(e, k, d[k])
+    b = m.BondParams()
+    b.type, b.r, b.lc0, b.lc1 = m.BondType.Edge, 2, d["bond_lc0"], d["bond_lc1"]
+    e.bond_params = b
+    r = m.SurfaceRepulsionParams()
+    r.n_search, r.rlist, r.lc1, r.r, r.exclusion_level = "cell-list", d["rep_rlist"], d["rep_lc1"], 2, 0
+    e.repulse_params = r
+    return e
+
+
+def system(core, level=2, seed=7):
+    v, f, lbar = M.jittered_icosphere(level, seed)
+    d = pdict(lbar)
+    mesh = core.TriMesh(v, f)
+    est = core.EnergyManager(mesh, core_params(core, d))
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, O.Params(**d))
+    return v, f, mesh, est, om, oe
+
+
+def ref_inputs(sg, v, f, om, oe):
+    nrm = sg.norm
+    norm = {"mu": np.asarray(nrm["mu"]), "sig": np.asarray(nrm["sig"]), "lbar": nrm["lbar"]}
+    fe, sp, nu = S.geometry(v, f, oe.vertex_properties(om), S.factors_of(oe, om), norm)
+    return fe, sp, nu, nrm["sigma_F"]
+
+
+def randomize(sg, seed):
+    rng = np.random.default_rng(seed)
+    sg.params = (np.asarray(sg.params) + rng.normal(size=len(sg.params)).astype(np.float32) * 0.05)
+
+
+def stages_ref(theta, fe, sp, nu):
+    """Input features and hidden states of the reference (bf16 operands)."""
+    p = S.unpack(torch.tensor(theta))
+    silu = torch.nn.functional.silu
+    f = torch.tensor(fe, dtype=torch.float32)
+    vi = torch.tensor([s[0] for s in sp])
+    ji = torch.tensor([s[1] for s in sp])
+    eps = torch.tensor([[s[2], s[3]] for s in sp], dtype=torch.float32)
+    H = silu(S.bf(f) @ S.bf(p["Win"]) + p["bin"])
+    out = [f.numpy(), H.numpy()]
+    for l in range(S.LAYERS):
+        P = S.bf(H) @ S.bf(p[f"U{l}"])
+        A = P[ji] + eps[:, 0:1] * p[f"E{l}"][0] + eps[:, 1:2] * p[f"E{l}"][1]
+        Mm = torch.zeros(len(fe), S.W).index_add(0, vi, silu(A)) / 6.0
+        H = H + silu(S.bf(torch.cat([H, Mm], 1)) @ S.bf(torch.cat([p[f"W{l}"], p[f"V{l}"]], 0)) + p[f"b{l}"])
+        out.append(H.numpy())
+    return out
+
+
+def test_forward_matches_reference(core):
+    for level in (2, 3):
+        v, f, mesh, est, om, oe = system(core, level)
+        sg = core.Surrogate(mesh, est, seed=1)
+        randomize(sg, level)
+        F = sg.forces()
+        fe, sp, nu, sF = ref_inputs(sg, v, f, om, oe)
+        ref = stages_ref(np.asarray(sg.params), fe, sp, nu)
+        for which, r in enumerate(ref):
+            a = sg.activations(which)
+            e = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
+            print("level", level, "stage", which, "rel err", e, flush=True)
+        Fr = S.forward(torch.tensor(np.asarray(sg.params)), fe, sp, nu, sF).numpy()
+        err = np.linalg.norm(F - Fr) / np.linalg.norm(Fr)
+        print("level", level, "forward rel err", err, flush=True)
+        assert err <= 1e-2
+
+
+def test_backward_matches_autograd(core):
+    v, f, mesh, est, om, oe = system(core, 2, seed=9)
+    est.precision = "fp64"
+    sg = core.Surrogate(mesh, est, seed=2)
+    randomize(sg, 5)
+    loss_gpu, g_gpu = sg.loss_grad()
+    fe, sp, nu, sF = ref_inputs(sg, v, f, om, oe)
+    th = torch.tensor(np.asarray(sg.params), requires_grad=True)
+    L = S.loss(S.forward(th, fe, sp, nu, sF), oe.gradient(om), sF)
+    (g_ref,) = torch.autograd.grad(L, th)
+    g_ref = g_ref.numpy()
+    print("loss gpu", loss_gpu, "ref", float(L), flush=True)
+    assert abs(loss_gpu - float(L)) <= 2e-2 * float(L)
+    off = S.OFF
+    blocks = [("Win", off["Win"], off["bin"]), ("bin", off["bin"], off["U0"])]
+    for l in range(S.LAYERS):
+        for name, nxt in (("U", "W"), ("W", "V"), ("V", "E"), ("E", "b")):
+            blocks.append((f"{name}{l}", off[f"{name}{l}"], off[f"{nxt}{l}"]))
+        blocks.append((f"b{l}", off[f"b{l}"], off[f"U{l + 1}"] if l + 1 < S.LAYERS else off["ws"]))
+    blocks += [("ws", off["ws"], off["bs"]), ("wt", off["wt"], off["bt"])]
+    for name, a, b in blocks:
+        x, y = g_gpu[a:b], g_ref[a:b]
+        if np.linalg.norm(y) == 0:
+            continue
+        cos = float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y)))
+        print(name, "cos", cos, flush=True)
+        assert cos >= 0.99, name
+    rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
+    print("gradient rel err", rel, flush=True)
+    assert rel <= 5e-2
+
+
+def test_training_lowers_loss(core):
+    v, f, mesh, est, om, oe = system(core, 3, seed=11)
+    sg = core.Surrogate(mesh, est, seed=3)
+    d = core.DeviceHMC(mesh, est)
+    held = None
+    for s in range(24):
+        d.step(mass=1.0, time_step=1e-3, num_integration_steps=5, temperature=1.0, seed=21, step=s)
+        if s == 23:
+            held = mesh.points().copy()
+        else:
+            sg.add_snapshot()
+    assert sg.num_snapshots() == 23
+    l_hold0, _ = sg.loss_grad()
+    first = sg.train(epochs=1, lr=1e-3, seed=1)
+    last = sg.train(epochs=40, lr=1e-3, seed=2)
+    assert np.array_equal(mesh.points(), held)    # positions restored
+    l_hold1, _ = sg.loss_grad()
+    print("train loss", first, "->", last, "held-out", l_hold0, "->", l_hold1, flush=True)
+    assert last < 0.3 * first
+    assert l_hold1 < 0.5 * l_hold0
+
+
+def test_surrogate_hmc_uses_exact_energies(core):
+    v, f, mesh, est, om, oe = system(core, 3, seed=13)
+    est.precision = "fp64"
+    sg = core.Surrogate(mesh, est, seed=4)
+    d = core.DeviceHMC(mesh, est)
+    for s in range(4):
+        x0 = mesh.points().copy()
+        e0 = est.energy(mesh)
+        d.step(mass=1.0, time_step=5e-4, num_integration_steps=6, temperature=1.0, seed=5, step=s, surrogate=sg)
+        r = d.result()
+        assert r["energy0"] == e0
+        x = mesh.points().copy()
+        if r["accepted"]:
+            assert est.energy(mesh) == r["energy1"]
+        else:
+            assert np.array_equal(x, x0)
+        print(s, r["accepted"], r["dh"], flush=True)

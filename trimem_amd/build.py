@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrimem_hip.so")
 
-HIP_SOURCES = ["tm_capi.hip", "tm_energy.hip", "tm_nlist.hip", "tm_flips.hip", "tm_hmc.hip"]
+HIP_SOURCES = ["tm_capi.hip", "tm_energy.hip", "tm_nlist.hip", "tm_flips.hip", "tm_hmc.hip", "tm_surrogate.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -336,6 +336,15 @@ struct EnergyManager {
     }
 };
 
+// The surrogate force network (tm_surrogate_*).
+struct Surrogate {
+    tm_surrogate* h = nullptr;
+    int nv = 0;
+    Surrogate(TriMesh& m, EnergyManager& e, uint64_t seed) : nv(m.nv()) { check(tm_surrogate_create(m.h, e.h, seed, &h)); }
+    Surrogate(const Surrogate&) = delete;
+    ~Surrogate() { tm_surrogate_destroy(h); }
+};
+
 // One HMC step of trimem.mc.hmc (hmc.py:133-158) run on the device; the
 // Python samplers in trimem_amd.mc drive it.
 struct DeviceHMC {
@@ -353,21 +362,23 @@ struct DeviceHMC {
     ~DeviceHMC() { tm_hmc_destroy(h); }
 
     void step(double mass, double time_step, int num_integration_steps, double temperature, py::object momenta,
-              double uniform, uint64_t seed, uint64_t step, double time_step_sq)
+              double uniform, uint64_t seed, uint64_t step, double time_step_sq, Surrogate* surrogate)
     {
         tm_hmc_params p{};
         p.mass = mass;
         p.time_step = time_step;
         p.time_step_sq = time_step_sq;
         p.num_integration_steps = num_integration_steps;
-        if (momenta.is_none()) {
-            check(tm_hmc_step(h, &p, temperature, nullptr, uniform, seed, step));
-            return;
+        const double* mp = nullptr;
+        py::array_t<double, py::array::c_style | py::array::forcecast> a;
+        if (!momenta.is_none()) {
+            a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(momenta);
+            if (!a || a.ndim() != 2 || a.shape(0) != nv || a.shape(1) != 3)
+                throw std::runtime_error("momenta must have shape (n_vertices, 3)");
+            mp = a.data();
         }
-        auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(momenta);
-        if (!a || a.ndim() != 2 || a.shape(0) != nv || a.shape(1) != 3)
-            throw std::runtime_error("momenta must have shape (n_vertices, 3)");
-        check(tm_hmc_step(h, &p, temperature, a.data(), uniform, seed, step));
+        if (surrogate) check(tm_hmc_step_surrogate(h, surrogate->h, &p, temperature, mp, uniform, seed, step));
+        else check(tm_hmc_step(h, &p, temperature, mp, uniform, seed, step));
     }
     py::dict result()
     {
@@ -690,8 +701,79 @@ PYBIND11_MODULE(core, m)
              py::keep_alive<1, 3>())
         .def("step", &DeviceHMC::step, py::kw_only(), py::arg("mass"), py::arg("time_step"), py::arg("num_integration_steps"),
             py::arg("temperature"), py::arg("momenta") = py::none(), py::arg("uniform") = -1.0,
-            py::arg("seed") = 0, py::arg("step") = 0, py::arg("time_step_sq") = 0.0)
+            py::arg("seed") = 0, py::arg("step") = 0, py::arg("time_step_sq") = 0.0,
+            py::arg("surrogate") = nullptr)
         .def("result", &DeviceHMC::result);
+
+    py::class_<Surrogate>(m, "Surrogate")
+        .def(py::init([](TriMesh& t, EnergyManager& e, uint64_t seed) { return std::make_unique<Surrogate>(t, e, seed); }),
+             py::arg("mesh"), py::arg("estore"), py::arg("seed") = 0, py::keep_alive<1, 2>(), py::keep_alive<1, 3>())
+        .def_property(
+            "params",
+            [](Surrogate& s) {
+                int n = 0;
+                check(tm_surrogate_params(s.h, nullptr, &n));
+                py::array_t<float> out(n);
+                check(tm_surrogate_params(s.h, out.mutable_data(), &n));
+                return out;
+            },
+            [](Surrogate& s, py::array_t<float, py::array::c_style | py::array::forcecast> p) {
+                int n = 0;
+                check(tm_surrogate_params(s.h, nullptr, &n));
+                if ((int)p.size() != n) throw std::runtime_error("wrong number of parameters");
+                check(tm_surrogate_set_params(s.h, p.data()));
+            })
+        .def_property_readonly("norm",
+                               [](Surrogate& s) {
+                                   py::array_t<double> mu(8), sig(8);
+                                   double lbar = 0, sf = 0;
+                                   check(tm_surrogate_norm(s.h, mu.mutable_data(), sig.mutable_data(), &lbar, &sf));
+                                   py::dict d;
+                                   d["mu"] = mu;
+                                   d["sig"] = sig;
+                                   d["lbar"] = lbar;
+                                   d["sigma_F"] = sf;
+                                   return d;
+                               })
+        .def("forces",
+             [](Surrogate& s) {
+                 py::array_t<double> out({(size_t)s.nv, (size_t)3});
+                 check(tm_surrogate_forces(s.h, out.mutable_data()));
+                 return out;
+             })
+        .def("loss_grad",
+             [](Surrogate& s) {
+                 int n = 0;
+                 check(tm_surrogate_params(s.h, nullptr, &n));
+                 py::array_t<float> g(n);
+                 double loss = 0;
+                 check(tm_surrogate_loss_grad(s.h, &loss, g.mutable_data()));
+                 return py::make_tuple(loss, g);
+             })
+        .def(
+            "activations",
+            [](Surrogate& s, int which) {
+                const size_t w = which == 0 ? 16 : 64;
+                py::array_t<float> out({(size_t)s.nv, w});
+                check(tm_surrogate_activations(s.h, which, out.mutable_data()));
+                return out;
+            },
+            py::arg("which"))
+        .def("add_snapshot", [](Surrogate& s) { check(tm_surrogate_add_snapshot(s.h)); })
+        .def("num_snapshots",
+             [](Surrogate& s) {
+                 int n = 0;
+                 check(tm_surrogate_num_snapshots(s.h, &n));
+                 return n;
+             })
+        .def(
+            "train",
+            [](Surrogate& s, int epochs, float lr, uint64_t seed) {
+                double loss = 0;
+                check(tm_surrogate_train(s.h, epochs, lr, seed, &loss));
+                return loss;
+            },
+            py::arg("epochs"), py::arg("lr") = 1e-3f, py::arg("seed") = 0);
 
     py::class_<Replicas>(m, "Replicas")
         .def(py::init([](const std::vector<TriMesh*>& meshes, const std::vector<EnergyManager*>& estores) {

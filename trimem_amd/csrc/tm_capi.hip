@@ -7,6 +7,7 @@
 // parameter arithmetic, and copying inputs and results.
 #include "../../include/trimem_hip.h"
 #include "tm_internal.h"
+#include "tm_surrogate.h"
 
 #include <hip/hip_runtime.h>
 
@@ -16,6 +17,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -1778,6 +1781,444 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
     S.drop_graph();
     set_group_shape(S, best);
     if (out) *out = to_c(best);
+    API_END
+}
+
+}    // extern "C"
+
+// ---- surrogate force network ----
+struct tm_surrogate {
+    tm_mesh* mesh = nullptr;
+    tm_energy* em = nullptr;
+    int nv = 0;
+    SgNorm norm{};
+    float sigma_F = 1.0f;
+    DevBuf<float> theta, grad, m1, m2;
+    DevBuf<short> packed;
+    SgPacked pk{};
+    DevBuf<float> act;    // every activation and work array, carved below
+    SgBuffers b{};
+    DevBuf<float> geo_buf;
+    SgGeom geo{};
+    DevBuf<int> twin, err;
+    // the feature pass: properties without repulsion, in buffers of its own
+    DevBuf<double> q, gprops, vprops, partials, rep_v;
+    DevBuf<int> pair_cnt;
+    DevBuf<unsigned> counter;
+    DevBuf<Globals> globals;
+    DevBuf<double> neg_force, xsave;
+    DevBuf<float> losses;
+    std::vector<std::unique_ptr<DevBuf<double>>> snap_x, snap_g;
+    int adam_step = 0;
+};
+
+namespace {
+
+void sg_alloc(tm_surrogate& S)
+{
+    const size_t N = (size_t)S.nv;
+    const size_t NW = N * kSgW;
+    S.theta.alloc(kSgParams);
+    S.grad.alloc(kSgParams);
+    S.m1.alloc(kSgParams);
+    S.m2.alloc(kSgParams);
+    HIPCHECK(hipMemsetAsync(S.m1.p, 0, sizeof(float) * kSgParams, g_rt.stream));
+    HIPCHECK(hipMemsetAsync(S.m2.p, 0, sizeof(float) * kSgParams, g_rt.stream));
+    // packed weights
+    const size_t np = (size_t)kSgW * kSgFeatPad + kSgLayers * (4 * kSgW * kSgW + 2 * kSgW * kSgW);
+    S.packed.alloc(np);
+    short* pp = S.packed.p;
+    S.pk.in_t = pp;
+    pp += kSgW * kSgFeatPad;
+    for (int l = 0; l < kSgLayers; l++) {
+        S.pk.u_t[l] = pp; pp += kSgW * kSgW;
+        S.pk.wv_t[l] = pp; pp += 2 * kSgW * kSgW;
+        S.pk.u_n[l] = pp; pp += kSgW * kSgW;
+        S.pk.w_n[l] = pp; pp += kSgW * kSgW;
+        S.pk.v_n[l] = pp; pp += kSgW * kSgW;
+    }
+    // activations
+    const size_t nact = N * kSgFeatPad + (2 * (kSgLayers + 1) + 2 * kSgLayers + 4) * NW + 3 * N + (N + 3) / 4 + 1 +
+                        sg_partial_floats(S.nv);
+    S.act.alloc(nact);
+    float* a = S.act.p;
+    S.b.f0 = a; a += N * kSgFeatPad;
+    for (int l = 0; l <= kSgLayers; l++) { S.b.Z[l] = a; a += NW; S.b.H[l] = a; a += NW; }
+    for (int l = 0; l < kSgLayers; l++) { S.b.P[l] = a; a += NW; S.b.M[l] = a; a += NW; }
+    S.b.dH = a; a += NW;
+    S.b.dZ = a; a += NW;
+    S.b.dP = a; a += NW;
+    S.b.dM = a; a += NW;
+    S.b.ghat = a; a += 3 * N;
+    S.b.loss_partial = a; a += (N + 3) / 4;
+    S.b.loss = a; a += 1;
+    S.b.partial = a;
+    S.geo_buf.alloc((size_t)kSgSlots * N * 5 + 3 * N);
+    S.geo.eps = S.geo_buf.p;
+    S.geo.ehat = S.geo_buf.p + (size_t)kSgSlots * N * 2;
+    S.geo.nu = S.geo_buf.p + (size_t)kSgSlots * N * 5;
+    S.twin.alloc((size_t)kSgSlots * N);
+    S.err.alloc(1);
+    S.q.alloc(N);
+    S.gprops.alloc(6 * N);
+    S.vprops.alloc(6 * N);
+    S.partials.alloc((size_t)pass1_blocks(S.nv) * 6);
+    S.rep_v.alloc(N);
+    S.pair_cnt.alloc(N);
+    S.counter.alloc(2);
+    S.globals.alloc(1);
+    S.neg_force.alloc(3 * N);
+    S.xsave.alloc(3 * N);
+}
+
+// properties without repulsion at the current positions into the
+// surrogate's own buffers, then the input features
+void sg_feature_pass(tm_surrogate& S)
+{
+    tm_mesh& M = *S.mesh;
+    tm_energy& E = *S.em;
+    EParams ep = eparams_of(E);
+    ep.rep_lc1 = 0.0;    // no repulsion query: the features do not use it
+    S.norm.ref[0] = ep.ref_area;
+    S.norm.ref[1] = ep.ref_volume;
+    S.norm.ref[2] = ep.ref_curvature;
+    PassBuffers pb{};
+    pb.q = S.q.p;
+    pb.gprops = S.gprops.p;
+    pb.pair_cnt = S.pair_cnt.p;
+    pb.rep_v = S.rep_v.p;
+    pb.partials = S.partials.p;
+    pb.counter = S.counter.p;
+    pb.globals = S.globals.p;
+    pb.list = M.generic.p;
+    pb.nlist = 0;
+    launch_pass1(M.view(), E.nlist->view(), ep, 1, pb, S.vprops.p, g_rt.stream);
+    sg_features(M.view(), S.vprops.p, S.globals.p, S.norm, S.geo, S.b.f0, S.err.p, g_rt.stream);
+    check_launch();
+}
+
+void sg_check_err(tm_surrogate& S)
+{
+    int e = 0;
+    d2h(&e, S.err.p, sizeof(int));
+    sync();
+    if (e) {
+        HIPCHECK(hipMemsetAsync(S.err.p, 0, sizeof(int), g_rt.stream));
+        throw TmError(TM_E_UNSUPPORTED, "surrogate needs a closed mesh with valences 3..10");
+    }
+}
+
+// the exact gradient at the current positions (energy manager buffers)
+void sg_exact_gradient(tm_surrogate& S)
+{
+    run_pass1(*S.em, *S.mesh, S.em->precision, nullptr);
+    run_pass2(*S.em, *S.mesh, S.em->precision);
+}
+
+}    // namespace
+
+extern "C" {
+
+int tm_surrogate_create(tm_mesh* mesh, tm_energy* em, uint64_t seed, tm_surrogate** out)
+{
+    API_BEGIN
+    require(mesh && em && out, "null argument");
+    require(em->nlist && em->nlist->nv == mesh->nv, "mesh does not match the energy manager");
+    tm_mesh& M = *mesh;
+    upload_points(M);
+    if (M.device_ahead) pull_points(M);    // host copies read below
+    refresh_host_topology(M);
+    require(M.nv > 0 && M.n_generic == 0, "surrogate needs a closed mesh");
+    auto S = std::make_unique<tm_surrogate>();
+    S->mesh = mesh;
+    S->em = em;
+    S->nv = M.nv;
+    sg_alloc(*S);
+    HIPCHECK(hipMemsetAsync(S->err.p, 0, sizeof(int), g_rt.stream));
+    sg_twin(M.view(), S->twin.p, g_rt.stream);
+    // mean edge length and raw features of the current state fix the
+    // normalisation
+    double lsum = 0.0;
+    long ne = 0;
+    for (int h = 0; h < M.nh; h += 2) {
+        const int a = M.perm[M.he_to[h]], b = M.perm[M.he_to[h + 1]];
+        const double* pa = M.points + 3 * a;
+        const double* pb = M.points + 3 * b;
+        lsum += std::sqrt((pa[0] - pb[0]) * (pa[0] - pb[0]) + (pa[1] - pb[1]) * (pa[1] - pb[1]) +
+                          (pa[2] - pb[2]) * (pa[2] - pb[2]));
+        ne++;
+    }
+    S->norm.lbar = ne ? lsum / ne : 1.0;
+    for (int i = 0; i < kSgFeat; i++) {
+        S->norm.mu[i] = 0.0;
+        S->norm.sig[i] = 1.0;
+    }
+    sg_feature_pass(*S);
+    sg_check_err(*S);
+    std::vector<float> f((size_t)M.nv * kSgFeatPad);
+    d2h(f.data(), S->b.f0, sizeof(float) * f.size());
+    sync();
+    for (int i = 0; i < kSgFeat; i++) {
+        double m = 0.0, v = 0.0;
+        for (int u = 0; u < M.nv; u++) m += f[(size_t)u * kSgFeatPad + i];
+        m /= M.nv;
+        for (int u = 0; u < M.nv; u++) {
+            const double d = f[(size_t)u * kSgFeatPad + i] - m;
+            v += d * d;
+        }
+        const double sd = std::sqrt(v / M.nv);
+        if (i >= 4 && i <= 6) {    // global deviations: already dimensionless
+            S->norm.mu[i] = 0.0;
+            S->norm.sig[i] = 1.0;
+        } else {
+            S->norm.mu[i] = m;
+            S->norm.sig[i] = sd > 1e-12 * std::fabs(m) + 1e-30 ? sd : 1.0;
+        }
+    }
+    // force scale: rms of the exact force now
+    sg_exact_gradient(*S);
+    std::vector<double> g((size_t)3 * M.nv);
+    d2h(g.data(), em->grad.p, sizeof(double) * g.size());
+    sync();
+    double ss = 0.0;
+    for (double x : g) ss += x * x;
+    S->sigma_F = (float)(ss > 0 ? std::sqrt(ss / g.size()) : 1.0);
+    // Glorot-uniform weights, small output layer, zero biases
+    std::vector<float> th(kSgParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int fan_in, int fan_out, int count, float gain) {
+        const float lim = gain * std::sqrt(6.0f / (float)(fan_in + fan_out));
+        std::uniform_real_distribution<float> U(-lim, lim);
+        for (int i = 0; i < count; i++) th[off + i] = U(rng);
+    };
+    fill(kSgOffWin, kSgFeat, kSgW, kSgFeatPad * kSgW, 1.0f);
+    for (int i = kSgFeat * kSgW; i < kSgFeatPad * kSgW; i++) th[kSgOffWin + i] = 0.0f;
+    for (int l = 0; l < kSgLayers; l++) {
+        fill(sg_off_U(l), kSgW, kSgW, kSgW * kSgW, 1.0f);
+        fill(sg_off_W(l), 2 * kSgW, kSgW, kSgW * kSgW, 0.5f);
+        fill(sg_off_V(l), 2 * kSgW, kSgW, kSgW * kSgW, 0.5f);
+        fill(sg_off_E(l), 2, kSgW, 2 * kSgW, 1.0f);
+    }
+    fill(kSgOffWs, kSgW, 1, kSgW, 0.1f);
+    fill(kSgOffWt, kSgW, 1, kSgW, 0.1f);
+    h2d(S->theta.p, th.data(), sizeof(float) * kSgParams);
+    sg_pack(S->theta.p, S->pk, g_rt.stream);
+    check_launch();
+    sync();
+    *out = S.release();
+    API_END
+}
+
+int tm_surrogate_destroy(tm_surrogate* sg)
+{
+    delete sg;
+    return TM_OK;
+}
+
+int tm_surrogate_params(tm_surrogate* sg, float* out, int* n)
+{
+    API_BEGIN
+    if (n) *n = kSgParams;
+    if (out) {
+        d2h(out, sg->theta.p, sizeof(float) * kSgParams);
+        sync();
+    }
+    API_END
+}
+
+int tm_surrogate_set_params(tm_surrogate* sg, const float* in)
+{
+    API_BEGIN
+    require(in != nullptr, "null argument");
+    h2d(sg->theta.p, in, sizeof(float) * kSgParams);
+    sg_pack(sg->theta.p, sg->pk, g_rt.stream);
+    check_launch();
+    sync();
+    API_END
+}
+
+int tm_surrogate_norm(tm_surrogate* sg, double* mu, double* sig, double* lbar, double* sigma_F)
+{
+    API_BEGIN
+    for (int i = 0; i < kSgFeat; i++) {
+        if (mu) mu[i] = sg->norm.mu[i];
+        if (sig) sig[i] = sg->norm.sig[i];
+    }
+    if (lbar) *lbar = sg->norm.lbar;
+    if (sigma_F) *sigma_F = sg->sigma_F;
+    API_END
+}
+
+int tm_surrogate_forces(tm_surrogate* sg, double* forces)
+{
+    API_BEGIN
+    tm_surrogate& S = *sg;
+    upload_points(*S.mesh);
+    sg_feature_pass(S);
+    sg_forward(S.mesh->view(), S.b, S.geo, S.theta.p, S.pk, S.sigma_F, nullptr, S.neg_force.p, g_rt.stream);
+    check_launch();
+    sg_check_err(S);
+    if (forces) {
+        std::vector<double> g((size_t)3 * S.nv);
+        download_rows(*S.mesh, S.neg_force.p, 3, g.data());
+        sync();
+        for (size_t i = 0; i < g.size(); i++) forces[i] = -g[i];
+    }
+    API_END
+}
+
+int tm_surrogate_loss_grad(tm_surrogate* sg, double* loss, float* grad)
+{
+    API_BEGIN
+    tm_surrogate& S = *sg;
+    upload_points(*S.mesh);
+    sg_exact_gradient(S);
+    sg_feature_pass(S);
+    sg_forward(S.mesh->view(), S.b, S.geo, S.theta.p, S.pk, S.sigma_F, S.em->grad.p, nullptr, g_rt.stream);
+    sg_backward(S.mesh->view(), S.b, S.geo, S.theta.p, S.pk, S.twin.p, S.grad.p, g_rt.stream);
+    check_launch();
+    sg_check_err(S);
+    float l = 0.0f;
+    d2h(&l, S.b.loss, sizeof(float));
+    if (grad) d2h(grad, S.grad.p, sizeof(float) * kSgParams);
+    sync();
+    if (loss) *loss = l;
+    API_END
+}
+
+int tm_surrogate_activations(tm_surrogate* sg, int which, float* out)
+{
+    API_BEGIN
+    tm_surrogate& S = *sg;
+    require(which >= 0 && which < 1 + (kSgLayers + 1) + kSgLayers, "unknown activation");
+    const float* src;
+    int w = kSgW;
+    if (which == 0) {
+        src = S.b.f0;
+        w = kSgFeatPad;
+    } else if (which <= kSgLayers + 1) {
+        src = S.b.H[which - 1];
+    } else {
+        src = S.b.M[which - kSgLayers - 2];
+    }
+    std::vector<float> a((size_t)S.nv * w);
+    d2h(a.data(), src, sizeof(float) * a.size());
+    sync();
+    for (int u = 0; u < S.nv; u++) {
+        const int i = S.mesh->iperm[u];
+        std::memcpy(out + (size_t)u * w, a.data() + (size_t)i * w, sizeof(float) * w);
+    }
+    API_END
+}
+
+int tm_surrogate_add_snapshot(tm_surrogate* sg)
+{
+    API_BEGIN
+    tm_surrogate& S = *sg;
+    upload_points(*S.mesh);
+    sg_exact_gradient(S);
+    auto x = std::make_unique<DevBuf<double>>();
+    auto g = std::make_unique<DevBuf<double>>();
+    x->alloc((size_t)3 * S.nv);
+    g->alloc((size_t)3 * S.nv);
+    HIPCHECK(hipMemcpyAsync(x->p, S.mesh->x.p, sizeof(double) * 3 * S.nv, hipMemcpyDeviceToDevice, g_rt.stream));
+    HIPCHECK(hipMemcpyAsync(g->p, S.em->grad.p, sizeof(double) * 3 * S.nv, hipMemcpyDeviceToDevice, g_rt.stream));
+    S.snap_x.push_back(std::move(x));
+    S.snap_g.push_back(std::move(g));
+    API_END
+}
+
+int tm_surrogate_num_snapshots(tm_surrogate* sg, int* n)
+{
+    *n = (int)sg->snap_x.size();
+    return TM_OK;
+}
+
+int tm_surrogate_train(tm_surrogate* sg, int epochs, float lr, uint64_t seed, double* mean_loss)
+{
+    API_BEGIN
+    tm_surrogate& S = *sg;
+    const int ns = (int)S.snap_x.size();
+    require(ns > 0, "no snapshots recorded");
+    require(epochs >= 0, "epochs must be >= 0");
+    tm_mesh& M = *S.mesh;
+    upload_points(M);
+    HIPCHECK(hipMemcpyAsync(S.xsave.p, M.x.p, sizeof(double) * 3 * S.nv, hipMemcpyDeviceToDevice, g_rt.stream));
+    S.losses.alloc(std::max(ns, 1));
+    std::vector<int> order(ns);
+    std::iota(order.begin(), order.end(), 0);
+    std::mt19937_64 rng(seed);
+    for (int ep = 0; ep < epochs; ep++) {
+        std::shuffle(order.begin(), order.end(), rng);
+        for (int i = 0; i < ns; i++) {
+            const int k = order[i];
+            HIPCHECK(hipMemcpyAsync(M.x.p, S.snap_x[k]->p, sizeof(double) * 3 * S.nv, hipMemcpyDeviceToDevice,
+                                    g_rt.stream));
+            sg_feature_pass(S);
+            sg_forward(M.view(), S.b, S.geo, S.theta.p, S.pk, S.sigma_F, S.snap_g[k]->p, nullptr, g_rt.stream);
+            sg_backward(M.view(), S.b, S.geo, S.theta.p, S.pk, S.twin.p, S.grad.p, g_rt.stream);
+            sg_adam(kSgParams, S.theta.p, S.grad.p, S.m1.p, S.m2.p, lr, ++S.adam_step, g_rt.stream);
+            sg_pack(S.theta.p, S.pk, g_rt.stream);
+            HIPCHECK(hipMemcpyAsync(S.losses.p + i, S.b.loss, sizeof(float), hipMemcpyDeviceToDevice, g_rt.stream));
+            check_launch();
+        }
+    }
+    HIPCHECK(hipMemcpyAsync(M.x.p, S.xsave.p, sizeof(double) * 3 * S.nv, hipMemcpyDeviceToDevice, g_rt.stream));
+    std::vector<float> ls(ns);
+    d2h(ls.data(), S.losses.p, sizeof(float) * ns);
+    sync();
+    sg_check_err(S);
+    if (mean_loss) {
+        double a = 0.0;
+        for (float x : ls) a += x;
+        *mean_loss = epochs > 0 ? a / ns : 0.0;
+    }
+    API_END
+}
+
+// HMC step with the surrogate's forces along the trajectory and the exact
+// energies at its two ends (hmc.py:133-158 with the force replaced)
+int tm_hmc_step_surrogate(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* prm, double temperature,
+                          const double* momenta, double uniform, uint64_t seed, uint64_t step)
+{
+    API_BEGIN
+    require(hmc && sg && prm, "null argument");
+    require(hmc->model == TM_HMC_ENERGY && hmc->em == sg->em && hmc->mesh == sg->mesh,
+            "the HMC handle and the surrogate must share mesh and energy manager");
+    require(prm->num_integration_steps >= 0, "num_integration_steps must be >= 0");
+    tm_mesh& M = *hmc->mesh;
+    tm_energy& E = *hmc->em;
+    tm_surrogate& S = *sg;
+    if (!M.device_ahead) push_points(M);
+    const int n3 = hmc->n3;
+    const double mass = prm->mass, dt = prm->time_step;
+    const double dt2 = prm->time_step_sq != 0.0 ? prm->time_step_sq : dt * dt;
+    const int L = prm->num_integration_steps;
+    if (momenta) {
+        h2d(M.stage.p, momenta, sizeof(double) * n3);
+        launch_gather_rows(M.stage.p, M.d_perm.p, hmc->p.p, M.nv, 3, g_rt.stream);
+        check_launch();
+        sync();
+    }
+    launch_hmc_begin(n3, M.x.p, hmc->x0.p, hmc->p.p, M.d_perm.p, momenta ? 0 : 1, seed, step,
+                     std::sqrt(mass * temperature), hmc->kpart0.p, g_rt.stream);
+    HmcState* st = hmc->state.p;
+    run_pass1(E, M, E.precision, nullptr);    // exact E(x0), read by the first leap
+    for (int i = 0; i <= L; i++) {
+        sg_feature_pass(S);
+        sg_forward(M.view(), S.b, S.geo, S.theta.p, S.pk, S.sigma_F, nullptr, S.neg_force.p, g_rt.stream);
+        int mode = i == 0 ? kHmcFirst : kHmcKick;
+        mode |= i < L ? kHmcDrift : kHmcLast;
+        launch_hmc_leap(n3, M.x.p, hmc->p.p, hmc->a.p, S.neg_force.p, dt, dt2, mass, mode, E.globals.p, st,
+                        hmc->kpart1.p, g_rt.stream);
+        check_launch();
+    }
+    run_pass1(E, M, E.precision, nullptr);    // exact E(xn)
+    launch_hmc_accept(n3, hmc->kpart0.p, hmc->kpart1.p, hmc->x0.p, M.x.p, 0, E.globals.p, mass, temperature,
+                      uniform, seed, step, st, g_rt.stream);
+    launch_hmc_restore(n3, M.x.p, hmc->x0.p, st, g_rt.stream);
+    check_launch();
+    M.device_ahead = true;
     API_END
 }
 
