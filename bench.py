@@ -261,6 +261,81 @@ def replica_rates(precision, evals, hmc_steps, hip, stream):
                         "[10,100] ladder; HMC dt 1e-5, 10 leapfrog steps, exchange every 10 steps"}
 
 
+def ess(x):
+    """Effective sample size of a scalar chain (Geyer's initial monotone
+    sequence estimator of the integrated autocorrelation time)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    x = x - x.mean()
+    var = float(x @ x) / n
+    if n < 4 or var == 0.0:
+        return float(n)
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n] / (n * var)
+    tau, prev = -1.0, np.inf
+    for k in range(0, n - 1, 2):
+        g = acf[k] + acf[k + 1]
+        if g <= 0:
+            break
+        g = min(g, prev)
+        tau += 2.0 * g
+        prev = g
+    return n / max(tau, 1.0)
+
+
+def surrogate_rates(precision, level=6, n_train=120, epochs=15, chain=300, time_step=1e-5, L=10, lr=1e-3):
+    """Surrogate-driven against plain HMC on one mesh: training from
+    snapshots of the plain chain (GPU forward/backward kernels, Adam), the
+    held-out force error, and for equal-length chains from the same state
+    steps/s, acceptance and the effective sample size of the energy."""
+    from trimem_amd import core
+    v, f, lbar, params = workload(level)
+    mesh = core.TriMesh(v, f)
+    est = core.EnergyManager(mesh, core_params(core, params))
+    est.precision = precision
+    d = core.DeviceHMC(mesh, est)
+    kw = dict(mass=1.0, time_step=time_step, num_integration_steps=L, temperature=1.0)
+    sg = core.Surrogate(mesh, est, seed=1)
+    t0 = time.perf_counter()
+    for s_ in range(n_train):
+        d.step(seed=3, step=s_, **kw)
+        sg.add_snapshot()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = sg.train(epochs=epochs, lr=lr, seed=1)
+    t_train = time.perf_counter() - t0
+    # held-out state: a few more plain steps
+    for s_ in range(n_train, n_train + 5):
+        d.step(seed=3, step=s_, **kw)
+    F = sg.forces()
+    G = est.gradient(mesh)
+    force_err = float(np.linalg.norm(F + G) / np.linalg.norm(G))
+    start = mesh.points().copy()
+
+    def run(use_sg, seed):
+        mesh.points()[:] = start
+        est.energy(mesh)    # host positions are the start state again
+        trace, acc = [], 0
+        t0 = time.perf_counter()
+        for k in range(chain):
+            d.step(seed=seed, step=k, surrogate=sg if use_sg else None, **kw)
+            r = d.result()
+            acc += int(r["accepted"])
+            trace.append(r["energy1"] if r["accepted"] else r["energy0"])
+        t = time.perf_counter() - t0
+        e = ess(trace)
+        return {"steps_per_s": chain / t, "acceptance": acc / chain, "ess": e, "ess_per_s": e / t}
+
+    plain = run(False, 11)
+    surr = run(True, 11)
+    return {"workload": f"icosphere L{level} ({len(v)} vertices), benchmark parameters, HMC dt {time_step}, "
+                        f"{L} leapfrog steps, T 1; {n_train} training snapshots from the plain chain, "
+                        f"{epochs} epochs of Adam (lr {lr})",
+            "train_seconds": t_train, "data_seconds": t_data, "final_train_loss": loss,
+            "heldout_force_rel_err": force_err, "plain": plain, "surrogate": surr,
+            "ess_per_s_ratio": surr["ess_per_s"] / plain["ess_per_s"] if plain["ess_per_s"] > 0 else None}
+
+
 def capi_handle(obj):
     """The C handle behind a trimem_amd.core TriMesh / EnergyManager."""
     return obj._handle()
@@ -381,6 +456,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--surrogate", type=int, default=1,
+                    help="1: include the surrogate-vs-plain HMC block (L6), 0: skip")
     ap.add_argument("--replica-evals", type=int, default=40,
                     help="grouped evaluations of the 64-replica set timed for the replicas block (0: skip)")
     ap.add_argument("--hmc-steps", type=int, default=40,
@@ -391,14 +468,14 @@ def main():
     args = ap.parse_args()
     if args.summarize:
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
-               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0 --replica-evals 0",
+               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0",
                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + args.summarize[1] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0",
+               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0",
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + args.summarize[2] +
                " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline"]
         if len(args.summarize) > 3:
             cmd.append("rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d " +
-                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0")
+                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0")
         cmd.append("python3 bench.py --summarize " + " ".join(args.summarize))
         summarize(args.summarize[0], args.summarize[1], args.summarize[2], PROFILE_JSON,
                   os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"), cmd,
@@ -516,6 +593,8 @@ def main():
     }
     if args.hmc_steps > 0:
         out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))
+    if args.surrogate:
+        out["surrogate"] = surrogate_rates(args.precision)
     if args.replica_evals > 0:
         out["replicas"] = replica_rates(args.precision, args.replica_evals, args.hmc_steps, hip, stream)
     if not args.no_cpu_baseline:

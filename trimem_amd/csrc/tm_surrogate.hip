@@ -79,8 +79,8 @@ __global__ void k_sg_pack(const float* __restrict__ theta, SgPacked pk)
 
 // --------------------------------------------------------------- GEMMs -----
 // Y[n][c] = sum_k A[n][k] Bt[c][k] over K = KA + KB (A = [A1 | A2]), c < 64.
-// Block: 4 waves, 64 rows; wave w owns rows 16w..16w+15 and all 64 columns
-// (four 16x16 tiles).  MFMA 16x16x16 operand lanes: A lane i holds row i % 16,
+// Block: 4 waves, 64 rows, staged in LDS with the weights; wave w owns rows
+// 16w..16w+15 and all 64 columns (four 16x16 tiles).  MFMA 16x16x16 operand lanes: A lane i holds row i % 16,
 // k = 4 (i / 16) + 0..3; B lane i holds column i % 16, the same k; the
 // accumulator lane i holds rows 4 (i / 16) + 0..3 of column i % 16.
 // Epilogues: kSgStore Y = acc; kSgAccum Y += acc; kSgAct Z = acc + bias and
@@ -93,10 +93,38 @@ __global__ void __launch_bounds__(256) k_sg_gemm(int N, const float* __restrict_
                                                  float* __restrict__ H)
 {
     constexpr int K = KA + KB;
+    constexpr int KP = K + 8;    // padded LDS row (bf16): spreads the fragment reads over banks
+    // the block's 64 rows of A and all of B, staged as bf16 with coalesced
+    // 16-byte global loads
+    __shared__ short s_a[64 * KP];
+    __shared__ short s_b[64 * KP];
+    const int row_base = blockIdx.x * 64;
+    for (int q = threadIdx.x; q < 64 * (K / 4); q += 256) {
+        const int r = q / (K / 4), k = 4 * (q % (K / 4));
+        const int row = row_base + r;
+        float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < N)
+            x = k < KA ? *reinterpret_cast<const float4*>(A1 + (size_t)row * KA + k)
+                       : *reinterpret_cast<const float4*>(A2 + (size_t)row * KB + (k - KA));
+        short* d = s_a + r * KP + k;
+        d[0] = to_bf16(x.x);
+        d[1] = to_bf16(x.y);
+        d[2] = to_bf16(x.z);
+        d[3] = to_bf16(x.w);
+    }
+    for (int q = threadIdx.x; q < 64 * (K / 4); q += 256) {
+        const int c = q / (K / 4), k = 4 * (q % (K / 4));
+        const short4 x = *reinterpret_cast<const short4*>(Bt + c * K + k);
+        short* d = s_b + c * KP + k;
+        d[0] = x.x;
+        d[1] = x.y;
+        d[2] = x.z;
+        d[3] = x.w;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row0 = blockIdx.x * 64 + w * 16;
-    const int ar = row0 + (lane & 15);
     const int kq = 4 * (lane >> 4);
+    const short* sa = s_a + (w * 16 + (lane & 15)) * KP + kq;
     floatx4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = floatx4{0, 0, 0, 0};
@@ -104,21 +132,17 @@ __global__ void __launch_bounds__(256) k_sg_gemm(int N, const float* __restrict_
     for (int k0 = 0; k0 < K; k0 += 16) {
         bf16x4 a;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int k = k0 + kq + i;
-            float x = 0.0f;
-            if (ar < N) x = k < KA ? A1[(size_t)ar * KA + k] : A2[(size_t)ar * KB + (k - KA)];
-            a[i] = to_bf16(x);
-        }
+        for (int i = 0; i < 4; i++) a[i] = sa[k0 + i];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const int col = 16 * t + (lane & 15);
+            const short* sb = s_b + (16 * t + (lane & 15)) * KP + kq + k0;
             bf16x4 b;
 #pragma unroll
-            for (int i = 0; i < 4; i++) b[i] = Bt[col * K + k0 + kq + i];
+            for (int i = 0; i < 4; i++) b[i] = sb[i];
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
         }
     }
+    const int row0 = row_base + w * 16;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int col = 16 * t + (lane & 15);
@@ -209,15 +233,22 @@ __global__ void __launch_bounds__(256) k_sg_colsum(int N, int rows_per_block, co
     if (w == 0) partial[(size_t)blockIdx.x * kSgW + lane] = s[0][lane] + s[1][lane] + s[2][lane] + s[3][lane];
 }
 
-// out[i] (+)= scale * sum_b partial[b][i], blocks in order
-__global__ void k_sg_reduce(int nblocks, int count, const float* __restrict__ partial, float scale, int accumulate,
-                            float* __restrict__ out)
+// out[i] (+)= scale * sum_b partial[b][i]: one block per output element,
+// each thread a fixed stride of the partials, then a fixed tree
+__global__ void __launch_bounds__(256) k_sg_reduce(int nblocks, int count, const float* __restrict__ partial,
+                                                   float scale, int accumulate, float* __restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+    __shared__ float s[256];
+    const int i = blockIdx.x;
     float a = 0.0f;
-    for (int b = 0; b < nblocks; b++) a += partial[(size_t)b * count + i];
-    out[i] = accumulate ? out[i] + scale * a : scale * a;
+    for (int b = threadIdx.x; b < nblocks; b += 256) a += partial[(size_t)b * count + i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = accumulate ? out[i] + scale * s[0] : scale * s[0];
 }
 
 // ------------------------------------------------------------ features -----
@@ -294,23 +325,24 @@ __global__ void k_sg_twin(MeshView m, int* __restrict__ twin)
 }
 
 // ------------------------------------------------------------- messages -----
-// One wave per vertex, lane = channel.
+// One wave per vertex (grid-stride), lane = channel.
 __global__ void __launch_bounds__(256) k_sg_message(MeshView m, const float* __restrict__ P,
                                                     const float* __restrict__ eps, const float* __restrict__ E,
                                                     float* __restrict__ M)
 {
-    const int v = blockIdx.x * 4 + (threadIdx.x >> 6), c = threadIdx.x & 63;
-    if (v >= m.nv) return;
+    const int c = threadIdx.x & 63;
     const int N = m.nv;
-    const int n = meta_valence(m.meta[v]);
     const float e0 = E[c], e1 = E[kSgW + c];
-    float acc = 0.0f;
-    for (int k = 0; k < n; k++) {
-        const int j = m.ring[k * N + v];
-        const int s = k * N + v;
-        acc += silu(P[(size_t)j * kSgW + c] + eps[2 * s] * e0 + eps[2 * s + 1] * e1);
+    for (int v = blockIdx.x * 4 + (threadIdx.x >> 6); v < N; v += gridDim.x * 4) {
+        const int n = meta_valence(m.meta[v]);
+        float acc = 0.0f;
+        for (int k = 0; k < n; k++) {
+            const int j = m.ring[k * N + v];
+            const int s = k * N + v;
+            acc += silu(P[(size_t)j * kSgW + c] + eps[2 * s] * e0 + eps[2 * s + 1] * e1);
+        }
+        M[(size_t)v * kSgW + c] = acc * (1.0f / 6.0f);
     }
-    M[(size_t)v * kSgW + c] = acc * (1.0f / 6.0f);
 }
 
 // Pull form of the message backward: the spoke (u -> j) feeds M_u through
@@ -324,9 +356,8 @@ __global__ void __launch_bounds__(256) k_sg_message_bwd(MeshView m, const int* _
 {
     __shared__ float s_e[4][2 * kSgW];
     const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
-    const int j = blockIdx.x * 4 + w;
     float de0 = 0.0f, de1 = 0.0f;
-    if (j < m.nv) {
+    for (int j = blockIdx.x * 4 + w; j < m.nv; j += gridDim.x * 4) {
         const int N = m.nv;
         const int n = meta_valence(m.meta[j]);
         const float e0 = E[c], e1 = E[kSgW + c];
@@ -383,9 +414,8 @@ __global__ void __launch_bounds__(256) k_sg_output(MeshView m, const float* __re
 {
     __shared__ float s_loss[4];
     const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
-    const int v = blockIdx.x * 4 + w;
     float lw = 0.0f;
-    if (v < m.nv) {
+    for (int v = blockIdx.x * 4 + w; v < m.nv; v += gridDim.x * 4) {
         const int N = m.nv;
         const int n = meta_valence(m.meta[v]);
         const float* ws = theta + kSgOffWs;
@@ -406,7 +436,7 @@ __global__ void __launch_bounds__(256) k_sg_output(MeshView m, const float* __re
             if (target) {
                 const float r = F[c] + (float)(target[3 * v + c] / sigma_F);
                 ghat[3 * v + c] = 2.0f * loss_scale * r;
-                lw = loss_scale * r * r;
+                lw += loss_scale * r * r;
             }
         }
     }
@@ -427,9 +457,8 @@ __global__ void __launch_bounds__(256) k_sg_output_bwd(MeshView m, const float* 
 {
     __shared__ float s_p[4][kSgOutGrad];
     const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
-    const int v = blockIdx.x * 4 + w;
     float dws = 0.0f, dbs = 0.0f, dwt = 0.0f, dbt = 0.0f;
-    if (v < m.nv) {
+    for (int v = blockIdx.x * 4 + w; v < m.nv; v += gridDim.x * 4) {
         const int N = m.nv;
         const int n = meta_valence(m.meta[v]);
         const float hv = H[(size_t)v * kSgW + c];
@@ -448,8 +477,8 @@ __global__ void __launch_bounds__(256) k_sg_output_bwd(MeshView m, const float* 
         }
         const float dt = gv0 * geo.nu[3 * v] + gv1 * geo.nu[3 * v + 1] + gv2 * geo.nu[3 * v + 2];
         dH[(size_t)v * kSgW + c] = theta[kSgOffWs + c] * acc + dt * theta[kSgOffWt + c];
-        dwt = dt * hv;
-        dbt = dt;
+        dwt += dt * hv;
+        dbt += dt;
     }
     // the parameter layout: w_s (64), b_s, w_t (64), b_t
     s_p[w][c] = dws;
@@ -479,6 +508,9 @@ __global__ void k_sg_adam(int n, float* __restrict__ theta, const float* __restr
 
 // ------------------------------------------------------------ launchers -----
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+// blocks of the one-wave-per-vertex kernels (4 vertices per block and
+// step, grid-stride beyond): a capped grid keeps their partials few
+int sg_vertex_blocks(int nv) { return std::min(cdiv(nv, 4), 4096); }
 
 // the largest reduction: K = 128 weight-gradient partials of every block
 size_t sg_partial_floats(int nv) { return (size_t)cdiv(nv, kSgRowsPerBlock) * 2 * kSgW * kSgW + (size_t)cdiv(nv, 4) * kSgOutGrad + 1024; }
@@ -515,14 +547,14 @@ static void wgrad(int N, const float* A1, const float* A2, const float* D, float
     const int nb = cdiv(N, rows);
     hipLaunchKernelGGL((k_sg_wgrad<KA, KB>), dim3(nb), dim3(256), 0, s, N, rows, A1, A2, D, partial);
     const int count = (KA + KB) * kSgW;
-    hipLaunchKernelGGL(k_sg_reduce, dim3(cdiv(count, 256)), dim3(256), 0, s, nb, count, partial, scale, 0, out);
+    hipLaunchKernelGGL(k_sg_reduce, dim3(count), dim3(256), 0, s, nb, count, partial, scale, 0, out);
 }
 
 static void colsum(int N, const float* D, float* partial, float* out, hipStream_t s)
 {
     const int nb = cdiv(N, kSgRowsPerBlock);
     hipLaunchKernelGGL(k_sg_colsum, dim3(nb), dim3(256), 0, s, N, kSgRowsPerBlock, D, partial);
-    hipLaunchKernelGGL(k_sg_reduce, dim3(1), dim3(kSgW), 0, s, nb, kSgW, partial, 1.0f, 0, out);
+    hipLaunchKernelGGL(k_sg_reduce, dim3(kSgW), dim3(256), 0, s, nb, kSgW, partial, 1.0f, 0, out);
 }
 
 // Forward: features already in b.f0; with a target also the loss gradient.
@@ -539,10 +571,11 @@ void sg_forward(const MeshView& m, const SgBuffers& b, const SgGeom& geo, const 
                                  b.H[l + 1], s);
     }
     const float loss_scale = 1.0f / (3.0f * (float)N);
-    hipLaunchKernelGGL(k_sg_output, dim3(cdiv(N, 4)), dim3(256), 0, s, m, b.H[kSgLayers], geo, theta, sigma_F,
+    hipLaunchKernelGGL(k_sg_output, dim3(sg_vertex_blocks(N)), dim3(256), 0, s, m, b.H[kSgLayers], geo, theta, sigma_F,
                        target, neg_force, b.ghat, b.loss_partial, loss_scale);
     if (target)
-        hipLaunchKernelGGL(k_sg_reduce, dim3(1), dim3(64), 0, s, cdiv(N, 4), 1, b.loss_partial, 1.0f, 0, b.loss);
+        hipLaunchKernelGGL(k_sg_reduce, dim3(1), dim3(256), 0, s, sg_vertex_blocks(N), 1, b.loss_partial, 1.0f, 0,
+                           b.loss);
 }
 
 // Backward after a forward with a target: parameter gradient into grad.
@@ -550,12 +583,12 @@ void sg_backward(const MeshView& m, const SgBuffers& b, const SgGeom& geo, const
                  const int* twin, float* grad, hipStream_t s)
 {
     const int N = m.nv;
-    const int nb4 = cdiv(N, 4);
+    const int nb4 = sg_vertex_blocks(N);
     const int cnt = N * kSgW;
     // output layer
     hipLaunchKernelGGL(k_sg_output_bwd, dim3(nb4), dim3(256), 0, s, m, b.H[kSgLayers], geo, theta, b.ghat, b.dH,
                        b.partial);
-    hipLaunchKernelGGL(k_sg_reduce, dim3(1), dim3(256), 0, s, nb4, kSgOutGrad, b.partial, 1.0f, 0,
+    hipLaunchKernelGGL(k_sg_reduce, dim3(kSgOutGrad), dim3(256), 0, s, nb4, kSgOutGrad, b.partial, 1.0f, 0,
                        grad + kSgOffWs);
     // message-passing layers, last to first; b.dH holds dL/dH_l on entry
     for (int l = kSgLayers - 1; l >= 0; l--) {
@@ -567,7 +600,7 @@ void sg_backward(const MeshView& m, const SgBuffers& b, const SgGeom& geo, const
         gemm<kSgW, 0, kSgStore>(N, b.dZ, nullptr, pk.v_n[l], nullptr, nullptr, b.dM, nullptr, s);
         hipLaunchKernelGGL(k_sg_message_bwd, dim3(nb4), dim3(256), 0, s, m, twin, b.P[l], geo.eps,
                            theta + sg_off_E(l), b.dM, b.dP, b.partial);
-        hipLaunchKernelGGL(k_sg_reduce, dim3(1), dim3(2 * kSgW), 0, s, nb4, 2 * kSgW, b.partial, 1.0f, 0,
+        hipLaunchKernelGGL(k_sg_reduce, dim3(2 * kSgW), dim3(256), 0, s, nb4, 2 * kSgW, b.partial, 1.0f, 0,
                            grad + sg_off_E(l));
         wgrad<kSgW, 0>(N, b.H[l], nullptr, b.dP, b.partial, grad + sg_off_U(l), 1.0f, s);
         gemm<kSgW, 0, kSgAccum>(N, b.dP, nullptr, pk.u_n[l], nullptr, nullptr, b.dH, nullptr, s);
