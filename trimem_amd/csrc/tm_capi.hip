@@ -117,6 +117,7 @@ struct tm_mesh {
     std::vector<int> he_to, he_next, he_prev, he_face, v_he, f_he;
     bool host_stale = false;     // device topology changed (flips)
     bool device_ahead = false;   // device positions moved by a device-resident step
+    unsigned topo_version = 0;   // bumped by every topology change (flips)
     bool has_boundary = false;
     // device state
     bool dev = false;
@@ -394,6 +395,8 @@ struct tm_nlist {
     DevBuf<int> cell_start, cell_verts, vert_cell, counts, fill, scan_tmp, adj_start, adj, adj_counts;
     DevBuf<int> cand_count, cand_start, cand;
     DevBuf<int2> vcand;
+    DevBuf<unsigned long long> excl_mask;
+    unsigned mask_topo = ~0u;    // topology version the masks were built for
     // distance-matrix scratch
     DevBuf<int> dm_counts, dm_off;
     DevBuf<double> dm_d;
@@ -406,6 +409,21 @@ struct tm_nlist {
 };
 
 namespace {
+
+// (Re)build the exclusion masks of a cell list when the mesh topology has
+// changed since they were built; the view carries them only while valid.
+void ensure_masks(tm_nlist& L, tm_mesh& M)
+{
+    if (L.kind != 0 || !L.excl_mask.p || M.nv == 0) return;
+    if (L.mask_topo != M.topo_version) {
+        NListView nv = L.v;
+        nv.excl_mask = nullptr;
+        launch_excl_mask(M.view(), nv, L.excl_mask.p, g_rt.stream);
+        check_launch();
+        L.mask_topo = M.topo_version;
+    }
+    L.v.excl_mask = L.excl_mask.p;
+}
 
 void scan(const int* in, int* out, int n, DevBuf<int>& tmp)
 {
@@ -506,6 +524,11 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         v.adj = L.adj.p;
     }
     L.v = v;
+    if (L.kind == 0 && n > 0) {
+        L.excl_mask.alloc((size_t)2 * n);
+        L.mask_topo = ~0u;
+        ensure_masks(L, M);
+    }
     sync();
 }
 
@@ -607,6 +630,7 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
         HIPCHECK(hipMemsetAsync(E.globals.p, 0, sizeof(Globals), g_rt.stream));
         return;
     }
+    ensure_masks(*E.nlist, M);
     launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), vprops, g_rt.stream, E.tune);
     check_launch();
 }
@@ -614,6 +638,7 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
 void run_pass2(tm_energy& E, tm_mesh& M, int mixed)
 {
     if (M.nv == 0) return;
+    ensure_masks(*E.nlist, M);
     launch_pass2(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), E.grad.p, g_rt.stream, E.tune);
     check_launch();
 }
@@ -1096,6 +1121,7 @@ static int flip_impl(tm_energy* em, tm_mesh* mesh, int nflips, const int* propos
     d2h(&ovf, mesh->flag.p, sizeof(int));
     sync();
     mesh->host_stale = true;
+    if (res[0] > 0 || res[2] != 0 || ovf) mesh->topo_version++;   // only accepted flips change topology
     if (res[2] == 1) throw TmError(TM_E_INVALID, "flip proposal out of range");
     if (res[2] == 2) throw TmError(TM_E_INVALID, "uniform stream exhausted");
     if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
@@ -1139,6 +1165,7 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
     d2h(&ovf, mesh->flag.p, sizeof(int));
     sync();
     mesh->host_stale = true;
+    if (st[7] > 0 || ovf) mesh->topo_version++;
     if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
     if (accepted) *accepted = (int)st[7];
     API_END
@@ -1355,6 +1382,7 @@ void refresh_replica_args(tm_replicas& S)
         tm_energy& E = *S.ems[r];
         ensure_device(M);
         ensure_energy_buffers(E, M);
+        ensure_masks(*E.nlist, M);
         EParams ep = eparams_of(E);
         if (S.has_ladder) ep.kappa_b = S.h_ladder[slot_of[r]];
         S.h_args[r] = pass_args(M.view(), E.nlist->view(), ep, buffers(E, M), nullptr, E.grad.p);
@@ -1391,6 +1419,13 @@ void replicas_device_points(tm_replicas& S)
 {
     for (tm_mesh* M : S.meshes)
         if (!M->device_ahead) push_points(*M);
+}
+
+// exclusion masks current for every replica's topology (the table holds
+// the mask buffers, whose addresses do not change)
+void replicas_masks(tm_replicas& S)
+{
+    for (int r = 0; r < S.R; r++) ensure_masks(*S.ems[r]->nlist, *S.meshes[r]);
 }
 
 void replicas_hmc_step(tm_replicas& S)
@@ -1440,6 +1475,7 @@ int tm_replicas_destroy(tm_replicas* rs)
 int tm_replicas_eval(tm_replicas* rs, int flags)
 {
     API_BEGIN
+    replicas_masks(*rs);
     if (flags & TM_EVAL_PROPS) launch_pass1_grouped(rs->group(), g_rt.stream);
     if (flags & TM_EVAL_GRAD) launch_pass2_grouped(rs->group(), g_rt.stream);
     check_launch();
@@ -1573,6 +1609,7 @@ int tm_replicas_hmc_run(tm_replicas* rs, int n_steps, int swap_every, int use_gr
     require(S.hmc_ready, "tm_replicas_hmc_setup first");
     require(n_steps >= 0, "n_steps must be >= 0");
     require(swap_every <= 0 || S.has_ladder, "exchange needs a ladder");
+    replicas_masks(S);
     if (use_graph && !S.graph_exec && n_steps > 0) {
         // the step reads every changing quantity (step counts, kappa_b, list
         // pointers) from device tables, so one capture serves all steps
@@ -1765,6 +1802,7 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
 {
     API_BEGIN
     tm_replicas& S = *rs;
+    replicas_masks(S);
     launch_pass1_grouped(S.group(), g_rt.stream);
     bool has_rep = false;
     for (const PassArgs& a : S.h_args) has_rep = has_rep || a.rep_v;

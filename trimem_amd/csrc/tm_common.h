@@ -102,6 +102,11 @@ struct NListView {
     // per cell; vcand[v] is the list of the build-time cell of v
     const int2* __restrict__ vcand;        // [N] range in cand of the build-time cell of each vertex
     const int* __restrict__ cand;
+    // per vertex, bit i set if the i-th candidate of its list is excluded
+    // (topology only; 2 x 64 bits, the first 128 candidates); null if not
+    // built. Read only by the vertex-group query of the energy passes, which
+    // run on a fixed topology; the flip kernels' walkers ignore it
+    const unsigned long long* __restrict__ excl_mask;
     // verlet list
     const int* __restrict__ adj_start;     // [N + 1]
     const int* __restrict__ adj;           // neighbour ids, ascending per vertex

@@ -329,7 +329,39 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
                             c[2] < nl.shape[2];
         const int flat = c[0] * nl.strides[0] + c[1] * nl.strides[1] + c[2] * nl.strides[2];
         if (inside && flat == vc) {
-            scan(nl.cand, vr.x, vr.y);
+            if (!nl.excl_mask) {
+                scan(nl.cand, vr.x, vr.y);
+                return;
+            }
+            // excluded candidates are skipped without a position load; the
+            // rest need only the distance test (the first 128 of the list)
+            const unsigned long long m0 = nl.excl_mask[2 * v], m1 = nl.excl_mask[2 * v + 1];
+            for (int q0 = vr.x + lane; q0 < vr.y; q0 += kBatch * kGroup) {
+                int j[kBatch];
+                int ex[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    const int q = q0 + u * kGroup;
+                    const int i = q - vr.x;
+                    ex[u] = i < 64 ? (int)((m0 >> i) & 1ull) : (i < 128 ? (int)((m1 >> (i - 64)) & 1ull) : -1);
+                    j[u] = (q < vr.y && ex[u] != 1) ? nl.cand[q] : -1;
+                }
+                d3 pj[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++)
+                    if (j[u] >= 0) pj[u] = load_p(m.x, j[u]);
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    if (j[u] < 0) continue;
+                    if (ex[u] < 0) {
+                        test(j[u], pj[u]);
+                        continue;
+                    }
+                    const d3 d = pi - pj[u];
+                    const double sq = sqnorm(d);
+                    if (sq <= dmax2 && sqrt(sq) < dmax) f(j[u], d);
+                }
+            }
             return;
         }
         // moved vertex: the 9 rows of x-consecutive cells of the stencil of
@@ -345,6 +377,46 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
     } else if (nl.kind == 1) {
         scan(nl.adj, nl.adj_start[v], nl.adj_start[v + 1]);
     }
+}
+
+// Exclusion masks of the stored stencil lists: bit i of vertex v is set if
+// the i-th candidate of the list of v's build-time cell is excluded from v
+// (excluded_fast, which depends on the ring tables only).  8-lane groups,
+// lane l takes candidates l, l + 8, ...; the group ORs its words.
+__global__ void __launch_bounds__(256) k_excl_mask(MeshView m, NListView nl, unsigned long long* __restrict__ mask)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = t / kGroup, lane = t % kGroup;
+    const bool active = v < m.nv;
+    unsigned long long w0 = 0ull, w1 = 0ull;
+    if (active) {
+        const int n = meta_valence(m.meta[v]);
+        const RingRegs R = load_ring(m, v, n);
+        const int2 vr = nl.vcand[v];
+        const int cnt = min(vr.y - vr.x, 128);
+        for (int i = lane; i < cnt; i += kGroup) {
+            const int j = nl.cand[vr.x + i];
+            if (excluded_fast(m, R, v, j, nl.exclusion)) {
+                if (i < 64) w0 |= 1ull << i;
+                else w1 |= 1ull << (i - 64);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < kGroup; o <<= 1) {
+        w0 |= __shfl_xor(w0, o, 64);
+        w1 |= __shfl_xor(w1, o, 64);
+    }
+    if (active && lane == 0) {
+        mask[2 * v] = w0;
+        mask[2 * v + 1] = w1;
+    }
+}
+
+void launch_excl_mask(const MeshView& m, const NListView& nl, unsigned long long* mask, hipStream_t s)
+{
+    if (m.nv <= 0) return;
+    hipLaunchKernelGGL(k_excl_mask, dim3((m.nv * kGroup + 255) / 256), dim3(256), 0, s, m, nl, mask);
 }
 
 // --------------------------------------------------------------- pass 1 ---

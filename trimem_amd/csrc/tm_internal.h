@@ -106,6 +106,9 @@ void launch_cell_sort_segments(const int* vert_cell, int n, const int* cell_star
 void launch_cand_count(const NListView& nl, int n, int* count, hipStream_t s);
 void launch_cand_fill(const NListView& nl, int n, const int* cand_start, int* cand, hipStream_t s);
 void launch_cand_vrange(const NListView& nl, int n, const int* cand_start, int2* vcand, hipStream_t s);
+// exclusion masks of the stored stencil lists (tm_energy.hip); nl needs
+// vcand and cand, its excl_mask is ignored
+void launch_excl_mask(const MeshView& m, const NListView& nl, unsigned long long* mask, hipStream_t s);
 // distance_matrix (cell list or verlet) in two phases: counts then fill
 void launch_dm_count(const MeshView& m, const NListView& nl, double dmax, int* counts, hipStream_t s);
 void launch_dm_fill(const MeshView& m, const NListView& nl, double dmax, const int* offsets, double* d, int* i,
