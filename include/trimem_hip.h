@@ -217,18 +217,26 @@ int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
 /* ---- launch shapes and the autotuner ----
  * No counterpart in the reference (OpenMP schedules).  The repulsion query
  * and the gradient pass write per-vertex values only, so their block size and
- * the number of candidates each lane keeps in flight change no result bit;
- * the autotuner times every shape on the caller's mesh (HIP events, the mean
- * of 5 launches after 2 untimed ones) and keeps the fastest. */
+ * the number of candidates each lane keeps in flight change no result bit.
+ * The query may also share one launch with the properties pass (rep_fused):
+ * its blocks then sum the repulsion column of the block partials in the
+ * order the properties pass uses, so this changes no bit either; it pays on
+ * small meshes, whose launches do not fill the GPU.  The autotuner times
+ * every shape on the caller's mesh (HIP events, the mean of 5 launches after
+ * 2 untimed ones), then the fused form against the best separate one, and
+ * keeps the fastest.  Replica groups always launch the query apart (their
+ * rep_fused reads 0). */
 typedef struct tm_tuning {
     int rep_threads;     /* repulsion query: 64, 128, 256 or 512 threads per block */
     int rep_batch;       /* candidates in flight per lane: 1, 2, 4 or 8 */
     int pass2_threads;   /* gradient pass: 64, 128, 256 or 512 */
-    int reserved;
+    int rep_fused;       /* 1: the query shares the launch of the properties pass
+                            (256 threads, rep_threads unused), 0: its own launch */
 } tm_tuning;
 int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t);
 int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out);
-/* ms_rep / ms_pass2 (may be NULL): the winners' times per launch */
+/* ms_rep / ms_pass2 (may be NULL): the winners' times per launch (ms_rep:
+ * of the fused properties launch when rep_fused is chosen) */
 int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_rep, double* ms_pass2);
 
 /* ---- replica groups: batched meshes, parallel tempering, graph replay ----

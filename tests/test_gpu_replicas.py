@@ -15,6 +15,8 @@ Pinned by per-replica equality with single-mesh runs:
     counter-based stream (restated below from its definition), and after a
     swap each replica's energy changes by (new kappa_b - old kappa_b) * B.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -172,22 +174,36 @@ def test_exchange_rule(core):
     assert 0 < st2["swaps_accepted"].sum() < 9    # both outcomes exercised
 
 
-SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64),
-          dict(rep_threads=512, rep_batch=8, pass2_threads=512),
-          dict(rep_threads=128, rep_batch=2, pass2_threads=128)]
+SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64, rep_fused=0),
+          dict(rep_threads=512, rep_batch=8, pass2_threads=512, rep_fused=1),
+          dict(rep_threads=128, rep_batch=2, pass2_threads=128, rep_fused=0),
+          dict(rep_threads=256, rep_batch=1, pass2_threads=256, rep_fused=1)]
+
+
+def vertex_properties(mesh, est):
+    """tm_energy_vertex_properties: the (N, 6) per-vertex properties."""
+    from trimem_amd import capi
+    fn = capi.lib().tm_energy_vertex_properties
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    out = np.zeros((mesh.n_vertices(), 6))
+    capi.check(fn(est._handle(), mesh._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_launch_shapes_change_no_bits(core, precision):
-    """Every repulsion / pass-2 launch shape, and the autotuner's choice, gives
-    the default shape's energy and gradient bits (exclusion 0 so that every
-    vertex has repulsion pairs)."""
+    """Every repulsion / pass-2 launch shape, the repulsion query inside or
+    apart from the properties launch, and the autotuner's choice give the
+    default shape's energy and gradient bits (exclusion 0 so that every vertex
+    has repulsion pairs)."""
     mesh, est = membrane(core, 4, 31, precision, excl=0)
-    e0, g0 = est.energy(mesh), est.gradient(mesh)
+    e0, g0, p0 = est.energy(mesh), est.gradient(mesh), vertex_properties(mesh, est)
+    assert np.all(p0[:, 5] > 0)
     for sh in SHAPES:
         est.tuning = sh
         assert est.tuning == sh
         assert est.energy(mesh) == e0 and np.array_equal(est.gradient(mesh), g0)
+        assert np.array_equal(vertex_properties(mesh, est), p0)
     best = est.autotune(mesh)
     print(precision, "autotuned", best, flush=True)
     assert best["rep_threads"] in (64, 128, 256, 512) and best["rep_batch"] in (1, 2, 4, 8)

@@ -404,6 +404,7 @@ py::dict tuning_dict(const tm_tuning& t)
     d["rep_threads"] = t.rep_threads;
     d["rep_batch"] = t.rep_batch;
     d["pass2_threads"] = t.pass2_threads;
+    d["rep_fused"] = t.rep_fused;
     return d;
 }
 
@@ -413,6 +414,7 @@ tm_tuning tuning_of(const py::dict& d)
     t.rep_threads = d["rep_threads"].cast<int>();
     t.rep_batch = d["rep_batch"].cast<int>();
     t.pass2_threads = d["pass2_threads"].cast<int>();
+    t.rep_fused = d.contains("rep_fused") ? d["rep_fused"].cast<int>() : 0;
     return t;
 }
 

@@ -1687,6 +1687,7 @@ Tuning from_c(const tm_tuning& c)
     t.rep_threads = c.rep_threads;
     t.rep_batch = c.rep_batch;
     t.p2_threads = c.pass2_threads;
+    t.rep_fused = c.rep_fused;
     return t;
 }
 
@@ -1696,6 +1697,7 @@ tm_tuning to_c(const Tuning& t)
     c.rep_threads = t.rep_threads;
     c.rep_batch = t.rep_batch;
     c.pass2_threads = t.p2_threads;
+    c.rep_fused = t.rep_fused;
     return c;
 }
 
@@ -1721,9 +1723,13 @@ double time_launches(F&& f, int reps)
 }
 
 // Try every shape of the repulsion query and of pass 2 with launch(t) and
-// keep the fastest of each; the two kernels are timed separately.
-template <class RepF, class P2F>
-Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, double* best_rep_ms, double* best_p2_ms)
+// keep the fastest of each; the two kernels are timed separately.  With
+// try_fused, the pass-1 launches (query + properties) of the best separate
+// shape are then timed against the fused launch at every batch, and the
+// faster form is kept (best_rep_ms is then the fused launch's time).
+template <class RepF, class P2F, class P1F>
+Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, P1F&& p1, bool try_fused, double* best_rep_ms,
+              double* best_p2_ms)
 {
     const int ths[4] = {64, 128, 256, 512}, bats[4] = {1, 2, 4, 8};
     Tuning best = base;
@@ -1734,6 +1740,7 @@ Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, double* best_rep_
                 Tuning t = best;
                 t.rep_threads = th;
                 t.rep_batch = ba;
+                t.rep_fused = 0;
                 const double ms = time_launches([&] { rep(t); }, 5);
                 if (ms < brep) {
                     brep = ms;
@@ -1741,6 +1748,27 @@ Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, double* best_rep_
                     best.rep_batch = ba;
                 }
             }
+    best.rep_fused = 0;
+    if (has_rep && try_fused) {
+        const double apart = time_launches([&] { p1(best); }, 5);
+        double bf = 1e30;
+        int bb = best.rep_batch;
+        for (int ba : bats) {
+            Tuning t = best;
+            t.rep_fused = 1;
+            t.rep_batch = ba;
+            const double ms = time_launches([&] { p1(t); }, 5);
+            if (ms < bf) {
+                bf = ms;
+                bb = ba;
+            }
+        }
+        if (bf < apart) {
+            best.rep_fused = 1;
+            best.rep_batch = bb;
+            brep = bf;
+        }
+    }
     for (int th : ths) {
         Tuning t = best;
         t.p2_threads = th;
@@ -1784,7 +1812,8 @@ int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_
     if (M.nv > 0)
         E.tune = search(
             E.tune, a.rep_v != nullptr, [&](const Tuning& t) { launch_rep(a, t, g_rt.stream); },
-            [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); }, ms_rep, ms_pass2);
+            [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); },
+            [&](const Tuning& t) { launch_pass1_body(a, E.precision, t, g_rt.stream); }, true, ms_rep, ms_pass2);
     if (out) *out = to_c(E.tune);
     API_END
 }
@@ -1794,7 +1823,9 @@ int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t)
     API_BEGIN
     require(t != nullptr && tuning_valid(from_c(*t)), "invalid launch shape");
     rs->drop_graph();
-    set_group_shape(*rs, from_c(*t));
+    Tuning g = from_c(*t);
+    g.rep_fused = 0;    // grouped launches keep the query apart
+    set_group_shape(*rs, g);
     API_END
 }
 
@@ -1815,7 +1846,8 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
     };
     const Tuning best = search(
         S.tune, has_rep, [&](const Tuning& t) { launch_rep_grouped(shaped(t), g_rt.stream); },
-        [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, ms_rep, ms_pass2);
+        [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, [](const Tuning&) {}, false, ms_rep,
+        ms_pass2);
     S.drop_graph();
     set_group_shape(S, best);
     if (out) *out = to_c(best);

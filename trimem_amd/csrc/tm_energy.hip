@@ -10,8 +10,8 @@
 //   pass 1  per vertex v: area, volume, curvature, tether, bending and
 //           repulsion from the faces (v, j_m, j_{m-1}) and edges (v, j_m) of
 //           its ring (mesh_properties.cpp:12-49); writes q_v = c_v / a_v and
-//           block partial sums of the six globals in FP64; the last block to
-//           finish sums the partials in a fixed order and writes the energy
+//           block partial sums of the six globals in FP64; a one-block
+//           kernel sums the partials in a fixed order and writes the energy
 //           and the penalty factors (kernel.h:19-122), so the result is
 //           deterministic and needs no float atomics.
 //   pass 2  per vertex v: the gather of mesh_properties.cpp:51-148 (spoke
@@ -110,14 +110,31 @@ __device__ __forceinline__ void block_sum6(double v[6], double* s_red /* [nwaves
     __syncthreads();
 }
 
-// Last block: fixed-order sum of all partials, energy and factors.
+// Fixed-order sum of all block partials, energy and factors.  The partials
+// are stored column by column (partials[k * nblocks + b], property k of
+// block b) so that each load of a wave reads one contiguous run.
 __device__ void finalize_globals(const double* __restrict__ partials, int nblocks, const EParams& p,
                                  Globals* g, double* s_red, double* s_out)
 {
+    // Each thread sums blocks tid, tid + blockDim, ... in that order; the rows
+    // of one sweep are all loaded before the first add, so their round trips
+    // overlap instead of following one another.
+    constexpr int kRows = 8;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += kRows * blockDim.x) {
+        double r[kRows][6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) v[k] += partials[b * 6 + k];
+        for (int u = 0; u < kRows; u++) {
+            const int b = b0 + u * blockDim.x;
+#pragma unroll
+            for (int k = 0; k < 6; k++) r[u][k] = b < nblocks ? partials[k * nblocks + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRows; u++)
+            if (b0 + u * (int)blockDim.x < nblocks) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) v[k] += r[u][k];
+            }
     }
     block_sum6(v, s_red, s_out);
     if (threadIdx.x == 0) {
@@ -467,7 +484,10 @@ __device__ __forceinline__ int find_replica(const int* __restrict__ block0, int 
     return lo;
 }
 
-template <class T>
+// kRepApart: the repulsion property (column 5 of the block partial and of
+// vprops) comes from a block of k_pass1_rep running beside this one, which
+// sums it in the same order as below; this block leaves that column alone.
+template <class T, bool kRepApart = false>
 __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
@@ -479,7 +499,7 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
     // repulsion sum of v from k_rep, loaded alongside meta
-    const double rv = (A.rep_v && active && lane == 0) ? A.rep_v[v] : 0.0;
+    const double rv = (!kRepApart && A.rep_v && active && lane == 0) ? A.rep_v[v] : 0.0;
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
     double area = 0, vol = 0, curv = 0, teth = 0;
@@ -505,9 +525,10 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
         } else {
             for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
         }
+        if (kRepApart) pr[5] = 0.0;
         s_q[threadIdx.x / kGroup] = pr[2] / pr[0];
         if (A.vprops) {
-            for (int k = 0; k < 6; k++) A.vprops[6 * v + k] = pr[k];
+            for (int k = 0; k < (kRepApart ? 5 : 6); k++) A.vprops[6 * v + k] = pr[k];
         }
     }
     block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q
@@ -517,7 +538,7 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
         const int i = threadIdx.x;
         if (i < 256 / kGroup && v0 + i < m.nv) A.q[v0 + i] = s_q[i];
     }
-    if (threadIdx.x < 6) A.partials[lb * 6 + threadIdx.x] = s_out[threadIdx.x];
+    if (threadIdx.x < (kRepApart ? 5 : 6)) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
 }
 
 template <class T>
@@ -542,9 +563,14 @@ __global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restric
 // chain of dependent loads (cell, candidate id, position, ring of the
 // candidate) and needs few registers, so it runs apart from the
 // register-heavy ring work of pass 1.
-template <int kThreads, int kBatch>
+// kPartial (256 threads, the 32 vertices of pass-1 block lb): also write
+// column 5 of that block's partial and of vprops, summed over the block's
+// group leaders exactly as pass1_body sums it, so the bits do not depend on
+// which of the two kernels computes it.
+template <int kThreads, int kBatch, bool kPartial = false>
 __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
 {
+    static_assert(!kPartial || kThreads == 256, "partials are per 256-thread pass-1 block");
     if (!A.rep_v) return;
     const MeshView& m = A.m;
     const EParams& p = A.p;
@@ -577,12 +603,36 @@ __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
         A.rep_v[v0 + i] = s_rep[i];
         A.pair_cnt[v0 + i] = s_cnt[i];
     }
+    if constexpr (kPartial) {
+        __shared__ double s_red[4 * 6];
+        __shared__ double s_out[6];
+        double pr[6] = {0, 0, 0, 0, 0, 0};
+        if (active && lane == 0) {
+            pr[5] = ring ? rep : A.gprops[6 * v + 5];
+            if (A.vprops) A.vprops[6 * v + 5] = pr[5];
+        }
+        block_sum6_leaders(pr, s_red, s_out);
+        if (threadIdx.x == 5) A.partials[5 * A.nblocks + lb] = s_out[5];
+    }
 }
 
 template <int kThreads, int kBatch>
 __global__ void __launch_bounds__(kThreads) k_rep(PassArgs A)
 {
     rep_body<kThreads, kBatch>(A, logical_block());
+}
+
+// Pass 1 and the repulsion query in one launch of 2 x nblocks blocks: logical
+// block 2b runs the query for the 32 vertices of pass-1 block b, 2b + 1 their
+// ring work.  The query waits on chains of loads and the ring work on the
+// vector ALU, so the two kinds of wave fill each other's idle cycles on the
+// same compute units (the pair shares an XCD and its L2).
+template <class T, int kBatch>
+__global__ void __launch_bounds__(256) k_pass1_rep(PassArgs A)
+{
+    const int lb = logical_block();
+    if (lb & 1) pass1_body<T, true>(A, lb >> 1);
+    else rep_body<256, kBatch, true>(A, lb >> 1);
 }
 
 template <int kThreads, int kBatch>
@@ -815,7 +865,8 @@ bool tuning_valid(const Tuning& t)
 {
     auto th = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512; };
     return th(t.rep_threads) && th(t.p2_threads) &&
-           (t.rep_batch == 1 || t.rep_batch == 2 || t.rep_batch == 4 || t.rep_batch == 8);
+           (t.rep_batch == 1 || t.rep_batch == 2 || t.rep_batch == 4 || t.rep_batch == 8) &&
+           (t.rep_fused == 0 || t.rep_fused == 1);
 }
 
 int tuned_blocks(int nv, int threads) { return nblocks(nblocks(nv * kGroup, threads), kXcds) * kXcds; }
@@ -873,6 +924,32 @@ static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs
     else p2_threads<double>(t.p2_threads, grouped, a, g, s);
 }
 
+template <class T>
+static void pass1_rep_go(int batch, const PassArgs& a, hipStream_t s)
+{
+    const dim3 grid(2 * a.nblocks), block(256);
+    switch (batch) {
+    case 1: hipLaunchKernelGGL((k_pass1_rep<T, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_pass1_rep<T, 2>), grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_pass1_rep<T, 8>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_pass1_rep<T, 4>), grid, block, 0, s, a); break;
+    }
+}
+
+void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
+{
+    if (a.m.nv == 0) return;
+    if (a.rep_v && t.rep_fused) {
+        if (mixed) pass1_rep_go<float>(t.rep_batch, a, s);
+        else pass1_rep_go<double>(t.rep_batch, a, s);
+        return;
+    }
+    launch_rep(a, t, s);
+    dim3 grid(a.nblocks), block(256);
+    if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
+}
+
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s)
 {
     if (a.rep_v && a.m.nv > 0) rep_dispatch(t, false, a, nullptr, s);
@@ -890,10 +967,7 @@ void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int 
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
-    launch_rep(a, t, s);
-    dim3 grid(a.nblocks), block(256);
-    if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
+    launch_pass1_body(a, mixed, t, s);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, a);
 }
 

@@ -45,16 +45,20 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
 // Launch shapes that do not change any result bit: the repulsion query and
 // pass 2 write per-vertex values only, and each lane visits its candidates in
 // the same order whatever the batch.  (Pass 1's blocks fix the order of the
-// global sums and stay at 256 threads.)
+// global sums and stay at 256 threads; fused, the query's blocks cover the
+// same 32 vertices as pass-1 blocks and sum their column the same way.)
 struct Tuning {
     int rep_threads = 256;    // 64, 128, 256 or 512
     int rep_batch = 4;        // candidates in flight per lane: 1, 2, 4 or 8
     int p2_threads = 256;     // 64, 128, 256 or 512
+    int rep_fused = 0;        // 1: the query shares the pass-1 launch (256 threads), 0: its own launch
 };
 bool tuning_valid(const Tuning& t);
 int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
 // the repulsion query and the pass-2 kernel alone (for timing)
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s);
+// the repulsion query and the pass-1 kernel (fused or apart), no finalize
+void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s);
 void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s);
 // R replicas: d_args in device memory, h_args its host copy (for the
 // launches of the boundary kernels), one prefix table of blocks per kernel
