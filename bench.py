@@ -139,21 +139,28 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
     from trimem_amd.mc.mesh import Mesh
 
     opts = {"time_step": 1e-5, "num_integration_steps": 10, "info_step": 0}
-    quiet = contextlib.redirect_stdout(io.StringIO())
-    mesh = Mesh(v, f)
-    est = core.EnergyManager(mesh.trimesh, core_params(core, params))
-    est.precision = precision
-    with quiet:
-        dev = H.DeviceMeshHMC(mesh, est, counter=H.get_step_counters(), options=opts, rng="device", seed=1)
-    dev.step()
-    est.update_repulsion(mesh.trimesh)
-    a0 = dev.result()["n_accepted"]
-    t0 = time.perf_counter()
-    for _ in range(dev_steps):
+
+    def device_chain(pair_cache):
+        mesh = Mesh(v, f)
+        est = core.EnergyManager(mesh.trimesh, core_params(core, params))
+        est.precision = precision
+        tun = est.tuning
+        tun["pair_cache"] = pair_cache
+        est.tuning = tun
+        with contextlib.redirect_stdout(io.StringIO()):
+            dev = H.DeviceMeshHMC(mesh, est, counter=H.get_step_counters(), options=opts, rng="device", seed=1)
         dev.step()
         est.update_repulsion(mesh.trimesh)
-    r = dev.result()
-    t_dev = time.perf_counter() - t0
+        a0 = dev.result()["n_accepted"]
+        t0 = time.perf_counter()
+        for _ in range(dev_steps):
+            dev.step()
+            est.update_repulsion(mesh.trimesh)
+        r = dev.result()
+        return time.perf_counter() - t0, r["n_accepted"] - a0
+
+    t_dev, acc = device_chain(1)
+    t_full, _ = device_chain(0)
 
     mesh2 = Mesh(v, f)
     est2 = core.EnergyManager(mesh2.trimesh, core_params(core, params))
@@ -171,9 +178,10 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
         host.step()
         funcs.callback(host.x, counter)
     t_host = time.perf_counter() - t0
-    return {"device_steps_per_s": dev_steps / t_dev, "dropin_steps_per_s": host_steps / t_host,
+    return {"device_steps_per_s": dev_steps / t_dev, "device_steps_per_s_full_query": dev_steps / t_full,
+            "dropin_steps_per_s": host_steps / t_host,
             "device_over_dropin": (dev_steps / t_dev) / (host_steps / t_host),
-            "device_acceptance": (r["n_accepted"] - a0) / dev_steps,
+            "device_acceptance": acc / dev_steps,
             "device_steps": dev_steps, "dropin_steps": host_steps,
             "trajectory": "dt 1e-5, 10 leapfrog steps (11 force evaluations), mass 1, T 1, "
                           "neighbour list rebuilt every step"}
@@ -357,12 +365,15 @@ def _find(d, suffix):
 def _short(name):
     if "copyBuffer" in name:
         return "copy"
-    for k in ("k_rep", "k_pass1", "k_pass2", "k_finalize", "k_generic_props", "k_generic_grad"):
+    for k in ("k_disp", "k_rep_cached", "k_rep_record", "k_rep", "k_pass1_apart", "k_pass1", "k_pass2",
+              "k_finalize", "k_generic_props", "k_generic_grad"):
         if k in name:
             return k + ("_fp64" if "<double>" in name else "")
     return None
 
 
+# the launches of pass 1 with the pair cache replaying (the default)
+PASS1_KERNELS = ("k_disp", "k_rep_cached", "k_pass1_apart", "k_finalize")
 SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
                "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU")
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4    # 1024 SIMDs, one wave64 VALU op per 4 cycles at 2.4 GHz
@@ -427,7 +438,7 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
         lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
                      f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
                      f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
-    p1 = [k for k in ("k_rep", "k_pass1", "k_finalize") if k in stats]
+    p1 = [k for k in PASS1_KERNELS if k in stats]
     if p1:
         lines += ["", f"Pass 1 ({' + '.join(p1)}): {sum(stats[k]['avg_us'] for k in p1):.2f} us, "
                       f"{sum(kernels.get(k, {}).get('hbm_bytes_per_dispatch', 0.0) for k in p1):.0f} HBM bytes."]
@@ -542,6 +553,23 @@ def main():
     capi.check(L.tm_energy_eval_device(em, mesh, both))
     capi.check(L.tm_sync())
 
+    # the same evaluations with the pair cache off (the full cell-list query
+    # every time), for comparison; the headline keeps the default
+    tun = capi.tm_tuning()
+    capi.check(L.tm_energy_get_tuning(em, ctypes.byref(tun)))
+    tun.pair_cache = 0
+    capi.check(L.tm_energy_set_tuning(em, ctypes.byref(tun)))
+    for _ in range(args.warmup):
+        L.tm_energy_eval_device(em, mesh, both)
+    hip.record(e0, stream)
+    for _ in range(args.steps):
+        L.tm_energy_eval_device(em, mesh, both)
+    hip.record(e1, stream)
+    capi.check(L.tm_sync())
+    full_query_evals_per_s = args.steps / (hip.elapsed_ms(e0, e1) / 1e3)
+    tun.pair_cache = 1
+    capi.check(L.tm_energy_set_tuning(em, ctypes.byref(tun)))
+
     # algorithmic bytes per launch: pass 1 reads positions (24 B/vertex) and
     # the triangle table (12 B/face) and writes a_i, c_i (16 B/vertex); pass 2
     # reads the same plus a_i, c_i and writes the gradient (24 B/vertex); the
@@ -549,9 +577,10 @@ def main():
     b_pass1 = 24.0 * n + 12.0 * nf + 16.0 * n
     b_pass2 = b_pass1 + 24.0 * n
     t1, t2 = float(np.mean(p1)) * 1e-3, float(np.mean(p2)) * 1e-3
-    # pass 1 is three launches on the stream (repulsion query, ring
-    # properties, fixed-order global sums), timed together; pass 2 is one
-    phases = {"pass1": ("k_rep", "k_pass1", "k_finalize"), "pass2": ("k_pass2",)}
+    # pass 1 is four launches on the stream (pair-cache displacement check,
+    # repulsion query, ring properties, fixed-order global sums), timed
+    # together; pass 2 is one
+    phases = {"pass1": PASS1_KERNELS, "pass2": ("k_pass2",)}
     dom = ("pass2", b_pass2, t2) if t2 >= t1 else ("pass1", b_pass1, t1)
     achieved = dom[1] / dom[2] / 1e9
     eval_bytes = b_pass1 + b_pass2
@@ -590,6 +619,7 @@ def main():
                      "valu_issue_frac": valu_frac,
                      "algorithmic_bytes": dom[1]},
         "wall_s_timed": wall,
+        "full_query_evals_per_s": full_query_evals_per_s,
     }
     if args.hmc_steps > 0:
         out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))

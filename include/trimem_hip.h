@@ -224,14 +224,17 @@ int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
  * small meshes, whose launches do not fill the GPU.  The autotuner times
  * every shape on the caller's mesh (HIP events, the mean of 5 launches after
  * 2 untimed ones), then the fused form against the best separate one, and
- * keeps the fastest.  Replica groups always launch the query apart (their
- * rep_fused reads 0). */
+ * keeps the fastest.  Replica groups always launch the query apart and in
+ * full (their rep_fused and pair_cache read 0). */
 typedef struct tm_tuning {
     int rep_threads;     /* repulsion query: 64, 128, 256 or 512 threads per block */
     int rep_batch;       /* candidates in flight per lane: 1, 2, 4 or 8 */
     int pass2_threads;   /* gradient pass: 64, 128, 256 or 512 */
     int rep_fused;       /* 1: the query shares the launch of the properties pass
                             (256 threads, rep_threads unused), 0: its own launch */
+    int pair_cache;      /* 1 (default): the cell-list query replays recorded
+                            candidate lists while they are provably complete
+                            (single meshes; bit-identical), 0: full query */
 } tm_tuning;
 int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t);
 int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out);

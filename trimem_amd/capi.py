@@ -34,6 +34,11 @@ class tm_params(ctypes.Structure):
     ]
 
 
+class tm_tuning(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int) for k in
+                ("rep_threads", "rep_batch", "pass2_threads", "rep_fused", "pair_cache")]
+
+
 class tm_props(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
                 ("area", "volume", "curvature", "bending", "tethering", "repulsion")]
@@ -76,6 +81,8 @@ def lib():
         "tm_energy_device_gradient": (dp, [P]),
         "tm_energy_device_globals": (dp, [P]),
         "tm_energy_update_repulsion": (I, [P, P]),
+        "tm_energy_get_tuning": (I, [P, ctypes.POINTER(tm_tuning)]),
+        "tm_energy_set_tuning": (I, [P, ctypes.POINTER(tm_tuning)]),
         "tm_sync": (I, []),
         "tm_stream": (I, [ctypes.POINTER(P)]),
     }

@@ -405,6 +405,7 @@ py::dict tuning_dict(const tm_tuning& t)
     d["rep_batch"] = t.rep_batch;
     d["pass2_threads"] = t.pass2_threads;
     d["rep_fused"] = t.rep_fused;
+    d["pair_cache"] = t.pair_cache;
     return d;
 }
 
@@ -415,6 +416,7 @@ tm_tuning tuning_of(const py::dict& d)
     t.rep_batch = d["rep_batch"].cast<int>();
     t.pass2_threads = d["pass2_threads"].cast<int>();
     t.rep_fused = d.contains("rep_fused") ? d["rep_fused"].cast<int>() : 0;
+    t.pair_cache = d.contains("pair_cache") ? d["pair_cache"].cast<int>() : 1;
     return t;
 }
 

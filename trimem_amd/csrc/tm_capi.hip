@@ -397,6 +397,12 @@ struct tm_nlist {
     DevBuf<int2> vcand;
     DevBuf<unsigned long long> excl_mask;
     unsigned mask_topo = ~0u;    // topology version the masks were built for
+    // pair cache of the query (PairCache): recorded lists, valid for this
+    // list and topology once a recording launch has been queued
+    DevBuf<double> pc_x, pc_disp;
+    DevBuf<int> pc_cell, pc_ent;
+    DevBuf<unsigned> pc_cnt;
+    bool pc_ok = false;
     // distance-matrix scratch
     DevBuf<int> dm_counts, dm_off;
     DevBuf<double> dm_d;
@@ -421,6 +427,7 @@ void ensure_masks(tm_nlist& L, tm_mesh& M)
         launch_excl_mask(M.view(), nv, L.excl_mask.p, g_rt.stream);
         check_launch();
         L.mask_topo = M.topo_version;
+        L.pc_ok = false;    // exclusion may have changed
     }
     L.v.excl_mask = L.excl_mask.p;
 }
@@ -528,6 +535,14 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         L.excl_mask.alloc((size_t)2 * n);
         L.mask_topo = ~0u;
         ensure_masks(L, M);
+        L.pc_x.alloc((size_t)3 * n);
+        L.pc_cell.alloc(n);
+        L.pc_cnt.alloc(n);
+        L.pc_ent.alloc((size_t)kPairLanes * kPairCap * n);
+        L.pc_disp.alloc(kDispBlocks);
+        HIPCHECK(hipMemsetAsync(L.pc_cell.p, 0xFF, sizeof(int) * n, g_rt.stream));    // -1: no replay
+        HIPCHECK(hipMemsetAsync(L.pc_disp.p, 0x7F, sizeof(double) * kDispBlocks, g_rt.stream));    // ~1e306
+        L.pc_ok = false;
     }
     sync();
 }
@@ -619,6 +634,18 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.globals = E.globals.p;
     b.list = M.generic.p;
     b.nlist = M.n_generic;
+    tm_nlist& L = *E.nlist;
+    if (E.tune.pair_cache && L.kind == 0 && L.pc_x.p && L.nv == M.nv && E.params.rep_lc1 > 0) {
+        const double skin = kPairSkin * E.params.rep_lc1;
+        b.pc.x_snap = L.pc_x.p;
+        b.pc.cell = L.pc_cell.p;
+        b.pc.cnt = L.pc_cnt.p;
+        b.pc.ent = L.pc_ent.p;
+        b.pc.disp_part = L.pc_disp.p;
+        b.pc.skin = skin;
+        b.pc.limit2 = (kPairLimit * skin) * (kPairLimit * skin);
+        b.pc.force = L.pc_ok ? 0 : 1;
+    }
     return b;
 }
 
@@ -631,8 +658,10 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
         return;
     }
     ensure_masks(*E.nlist, M);
-    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, buffers(E, M), vprops, g_rt.stream, E.tune);
+    const PassBuffers b = buffers(E, M);
+    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, b, vprops, g_rt.stream, E.tune);
     check_launch();
+    if (b.pc.x_snap) E.nlist->pc_ok = true;    // recorded by this launch, or still valid
 }
 
 void run_pass2(tm_energy& E, tm_mesh& M, int mixed)
@@ -1385,7 +1414,9 @@ void refresh_replica_args(tm_replicas& S)
         ensure_masks(*E.nlist, M);
         EParams ep = eparams_of(E);
         if (S.has_ladder) ep.kappa_b = S.h_ladder[slot_of[r]];
-        S.h_args[r] = pass_args(M.view(), E.nlist->view(), ep, buffers(E, M), nullptr, E.grad.p);
+        PassBuffers b = buffers(E, M);
+        b.pc = PairCache{};    // the cache serves single-mesh launches only
+        S.h_args[r] = pass_args(M.view(), E.nlist->view(), ep, b, nullptr, E.grad.p);
         S.h_block0[r + 1] = S.h_block0[r] + S.h_args[r].nblocks;
     }
     S.total_blocks = S.h_block0[S.R];
@@ -1405,6 +1436,8 @@ void set_group_shape(tm_replicas& S, const Tuning& t)
         p2[r + 1] = p2[r] + tuned_blocks(S.meshes[r]->nv, t.p2_threads);
     }
     S.tune = t;
+    S.tune.rep_fused = 0;     // grouped launches keep the query apart
+    S.tune.pair_cache = 0;    // and run it in full
     S.total_rep = rep[S.R];
     S.total_p2 = p2[S.R];
     S.d_block0_rep.alloc(S.R + 1);
@@ -1688,6 +1721,7 @@ Tuning from_c(const tm_tuning& c)
     t.rep_batch = c.rep_batch;
     t.p2_threads = c.pass2_threads;
     t.rep_fused = c.rep_fused;
+    t.pair_cache = c.pair_cache;
     return t;
 }
 
@@ -1698,6 +1732,7 @@ tm_tuning to_c(const Tuning& t)
     c.rep_batch = t.rep_batch;
     c.pass2_threads = t.p2_threads;
     c.rep_fused = t.rep_fused;
+    c.pair_cache = t.pair_cache;
     return c;
 }
 
@@ -1807,13 +1842,17 @@ int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_
     tm_energy& E = *em;
     tm_mesh& M = *mesh;
     upload_points(M);
-    run_pass1(E, M, E.precision, nullptr);    // pass 2 reads q and the globals
+    // pass 2 reads q and the globals; the second pass 1 leaves the pair
+    // cache in its steady state (recorded, displacement measured)
+    run_pass1(E, M, E.precision, nullptr);
+    run_pass1(E, M, E.precision, nullptr);
     const PassArgs a = pass_args(M.view(), E.nlist->view(), eparams_of(E), buffers(E, M), nullptr, E.grad.p);
     if (M.nv > 0)
         E.tune = search(
             E.tune, a.rep_v != nullptr, [&](const Tuning& t) { launch_rep(a, t, g_rt.stream); },
             [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); },
-            [&](const Tuning& t) { launch_pass1_body(a, E.precision, t, g_rt.stream); }, true, ms_rep, ms_pass2);
+            [&](const Tuning& t) { launch_pass1_body(a, E.precision, t, g_rt.stream); },
+            a.pc.x_snap == nullptr, ms_rep, ms_pass2);    // with the cache the query is never fused
     if (out) *out = to_c(E.tune);
     API_END
 }
@@ -1823,9 +1862,7 @@ int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t)
     API_BEGIN
     require(t != nullptr && tuning_valid(from_c(*t)), "invalid launch shape");
     rs->drop_graph();
-    Tuning g = from_c(*t);
-    g.rep_fused = 0;    // grouped launches keep the query apart
-    set_group_shape(*rs, g);
+    set_group_shape(*rs, from_c(*t));
     API_END
 }
 

@@ -164,6 +164,7 @@ __device__ void finalize_globals(const double* __restrict__ partials, int nblock
 // round trips, and the group sums its per-slot terms with lane shuffles.
 constexpr int kGroup = 8;
 static_assert(kGroup == kRingPad, "exclusion tests read kGroup ring columns without the valence");
+static_assert(kGroup == kPairLanes && kPairCap <= 15, "pair-cache counts are 4 bits per lane");
 
 // Sums over the 8 lanes of a group with DPP lane moves (no LDS traffic):
 // xor 1 and xor 2 within quads, then each lane i of the group pairs with lane
@@ -396,6 +397,112 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
     }
 }
 
+// ------------------------------------------------------------ pair cache ---
+// (PairCache in tm_internal.h.)  Cell of v at position pi, or -1 outside the
+// grid.
+__device__ __forceinline__ int query_cell(const NListView& nl, d3 pi)
+{
+    int c[3];
+    cell_coords(nl, pi, c);
+    const bool inside = c[0] >= 0 && c[0] < nl.shape[0] && c[1] >= 0 && c[1] < nl.shape[1] && c[2] >= 0 &&
+                        c[2] < nl.shape[2];
+    return inside ? c[0] * nl.strides[0] + c[1] * nl.strides[1] + c[2] * nl.strides[2] : -1;
+}
+
+// Block-uniform: 2 to record (forced, or some vertex has moved skin / 2 or
+// more since the recording), 1 to replay.  Contains a barrier.
+__device__ __forceinline__ int pair_cache_mode(const PairCache& pc)
+{
+    if (pc.force) return 2;
+    __shared__ int s_mode;
+    if (threadIdx.x < 64) {
+        double mx = 0.0;
+        for (int i = threadIdx.x; i < kDispBlocks; i += 64) mx = fmax(mx, pc.disp_part[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+        if (threadIdx.x == 0) s_mode = mx < pc.limit2 ? 1 : 2;
+    }
+    __syncthreads();
+    return s_mode;
+}
+
+// The full query with radius dmax + skin, keeping each lane's non-excluded
+// candidates in its visit order; f sees the pairs within dmax, in the order
+// and with the test of the plain query.  Lane 0 stores the counts and the
+// cell the lists are valid in (-1 when a lane overflows).
+template <int kBatch, class F>
+__device__ __forceinline__ void record_neighbours(const MeshView& m, const NListView& nl, const PairCache& pc, int v,
+                                                  int n, d3 pi, double dmax, int lane, F&& f)
+{
+    int* e = pc.ent + ((size_t)v * kGroup + lane) * kPairCap;
+    const double dmax2 = dmax * dmax * (1.0 + 1e-15);
+    int rec = 0;
+    group_neighbours<kBatch>(m, nl, v, n, pi, dmax + pc.skin, lane, [&](int j, d3 d) {
+        if (rec < kPairCap) e[rec] = j;
+        rec++;
+        const double sq = sqnorm(d);
+        if (sq <= dmax2 && sqrt(sq) < dmax) f(j, d);
+    });
+    const int over = group_sum(rec > kPairCap ? 1 : 0);
+    const int word = group_sum(min(rec, kPairCap) << (4 * lane));
+    const int cell = query_cell(nl, pi);
+    if (lane == 0) {
+        pc.cnt[v] = (unsigned)word;
+        pc.cell[v] = over == 0 ? cell : -1;
+    }
+}
+
+// Replay of the recorded lists (false, and nothing visited, if v is not in
+// the cell they were recorded in).
+template <class F>
+__device__ __forceinline__ bool cached_neighbours(const MeshView& m, const NListView& nl, const PairCache& pc, int v,
+                                                  d3 pi, double dmax, int lane, F&& f)
+{
+    const int cell = pc.cell[v];
+    const unsigned word = pc.cnt[v];
+    if (cell < 0 || query_cell(nl, pi) != cell) return false;
+    const int k = (int)((word >> (4 * lane)) & 15u);
+    const int* e = pc.ent + ((size_t)v * kGroup + lane) * kPairCap;
+    int j[kPairCap];
+#pragma unroll
+    for (int i = 0; i < kPairCap; i++) j[i] = i < k ? e[i] : -1;
+    d3 pj[kPairCap];
+#pragma unroll
+    for (int i = 0; i < kPairCap; i++)
+        if (j[i] >= 0) pj[i] = load_p(m.x, j[i]);
+    const double dmax2 = dmax * dmax * (1.0 + 1e-15);
+#pragma unroll
+    for (int i = 0; i < kPairCap; i++) {
+        if (j[i] < 0) continue;
+        const d3 d = pi - pj[i];
+        const double sq = sqnorm(d);
+        if (sq <= dmax2 && sqrt(sq) < dmax) f(j[i], d);
+    }
+    return true;
+}
+
+// max |x - x_snap|^2 over the vertices, one partial per block (NaN counts as
+// infinite, which forces a recording)
+__global__ void __launch_bounds__(256) k_disp(const double* __restrict__ x, const double* __restrict__ xs, int n,
+                                              double* __restrict__ part)
+{
+    __shared__ double s_red[4];
+    double mx = 0.0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const double dx = x[3 * v] - xs[3 * v];
+        const double dy = x[3 * v + 1] - xs[3 * v + 1];
+        const double dz = x[3 * v + 2] - xs[3 * v + 2];
+        double q = dx * dx + dy * dy + dz * dz;
+        if (!(q == q)) q = INFINITY;
+        mx = fmax(mx, q);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+}
+
 // Exclusion masks of the stored stencil lists: bit i of vertex v is set if
 // the i-th candidate of the list of v's build-time cell is excluded from v
 // (excluded_fast, which depends on the ring tables only).  8-lane groups,
@@ -547,6 +654,13 @@ __global__ void __launch_bounds__(256) k_pass1(PassArgs A)
     pass1_body<T>(A, logical_block());
 }
 
+// pass 1 without the repulsion column (written by the query's kernel)
+template <class T>
+__global__ void __launch_bounds__(256) k_pass1_apart(PassArgs A)
+{
+    pass1_body<T, true>(A, logical_block());
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
                                                        int R)
@@ -620,6 +734,179 @@ template <int kThreads, int kBatch>
 __global__ void __launch_bounds__(kThreads) k_rep(PassArgs A)
 {
     rep_body<kThreads, kBatch>(A, logical_block());
+}
+
+// The repulsion query with the pair cache (PairCache), kVertsPerBlock
+// vertices per block of 256 threads.  Phase 1, one thread per vertex (the
+// first kVertsPerBlock threads): a vertex whose lists may be
+// replayed sums each lane's recorded pairs in that lane's order and combines
+// the eight lane sums in the order of group_sum ((l0+l1)+(l2+l3)) +
+// ((l4+l5)+(l6+l7)), so it gets the bits of the 8-lane query; the other ring
+// vertices (all of them in a recording launch) go to a block-local list.
+// Phase 2: the block's 32 lane groups run the full query (recording when the
+// mode says so) over that list.  Replays are the common case and want few,
+// full blocks; a recording launch the device decides on (some vertex moved
+// too far) is slower than k_rep, as each group takes 8 vertices in turn.
+constexpr int kVertsPerBlock = 256;
+template <int kBatch>
+__global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
+{
+    const MeshView& m = A.m;
+    const EParams& p = A.p;
+    const PairCache& pc = A.pc;
+    const int mode = pair_cache_mode(pc);    // block-uniform, 1 replay, 2 record
+    __shared__ int s_list[kVertsPerBlock];
+    __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (pr[5] of pass 1)
+    __shared__ int s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int base = logical_block() * kVertsPerBlock;
+    const int v = base + threadIdx.x;
+    if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
+    if (threadIdx.x < kVertsPerBlock && v < m.nv) {
+        const int meta = m.meta[v];
+        const bool ring = !is_generic(meta);
+        const d3 xv = load_p(m.x, v);
+        bool done = !ring;
+        double rep = 0.0;
+        int cnt = 0;
+        if (mode == 2) {
+            pc.x_snap[3 * v] = xv.x;
+            pc.x_snap[3 * v + 1] = xv.y;
+            pc.x_snap[3 * v + 2] = xv.z;
+            if (!ring) pc.cell[v] = -1;
+        } else if (ring) {
+            const int cell = pc.cell[v];
+            const unsigned word = pc.cnt[v];
+            if (cell >= 0 && query_cell(A.nl, xv) == cell) {
+                const double dmax = p.rep_lc1, dmax2 = dmax * dmax * (1.0 + 1e-15);
+                double ls[kGroup];
+#pragma unroll
+                for (int l = 0; l < kGroup; l++) {
+                    const int k = (int)((word >> (4 * l)) & 15u);
+                    const int* e = pc.ent + ((size_t)v * kGroup + l) * kPairCap;
+                    double acc = 0.0;
+                    for (int i = 0; i < k; i++) {
+                        const d3 d = xv - load_p(m.x, e[i]);
+                        const double sq = sqnorm(d);
+                        if (sq <= dmax2 && sqrt(sq) < dmax) {
+                            acc += rep_value(norm(d), p.rep_lc1, p.rep_r);
+                            cnt++;
+                        }
+                    }
+                    ls[l] = acc;
+                }
+                rep = ((ls[0] + ls[1]) + (ls[2] + ls[3])) + ((ls[4] + ls[5]) + (ls[6] + ls[7]));
+                done = true;
+            }
+        }
+        if (done) {
+            A.rep_v[v] = rep;
+            A.pair_cnt[v] = cnt;
+            s_val[threadIdx.x] = ring ? rep : A.gprops[6 * v + 5];
+        } else {
+            s_list[atomicAdd(&s_n, 1)] = v;
+        }
+    }
+    __syncthreads();
+    const int n_full = s_n;
+    const int g = threadIdx.x / kGroup, lane = threadIdx.x % kGroup;
+    for (int i = g; i < n_full; i += 256 / kGroup) {
+        const int u = s_list[i];
+        const d3 xu = load_p(m.x, u);
+        const int n = meta_valence(m.meta[u]);
+        double rep = 0.0;
+        int cnt = 0;
+        auto add = [&](int, d3 d) {
+            rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
+            cnt++;
+        };
+        if (mode == 2) record_neighbours<kBatch>(m, A.nl, pc, u, n, xu, p.rep_lc1, lane, add);
+        else group_neighbours<kBatch>(m, A.nl, u, n, xu, p.rep_lc1, lane, add);
+        rep = group_sum(rep);
+        cnt = group_sum(cnt);
+        if (lane == 0) {
+            A.rep_v[u] = rep;
+            A.pair_cnt[u] = cnt;
+            s_val[u - base] = rep;
+        }
+    }
+    __syncthreads();
+    if (A.vprops && threadIdx.x < kVertsPerBlock && v < m.nv) A.vprops[6 * v + 5] = s_val[threadIdx.x];
+    // column 5 of the pass-1 partials of the block's 32-vertex chunks, summed
+    // as block_sum6_leaders sums it in pass 1: within each 8-vertex wave
+    // ((g0+g1)+(g2+g3))+((g4+g5)+(g6+g7)), then the four waves in order
+    constexpr int kChunks = kVertsPerBlock / 32;
+    if (threadIdx.x < kChunks) {
+        const int c = threadIdx.x;
+        const double* g = s_val + 32 * c;
+        double a = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const double* q = g + 8 * w;
+            a += ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+        }
+        const int lb1 = base / 32 + c;
+        if (lb1 < A.nblocks) A.partials[5 * A.nblocks + lb1] = a;
+    }
+}
+
+// A recording launch the host asks for (new list or topology): the layout
+// of k_rep (32 vertices per block, one lane group each), recording every
+// ring vertex's lists and every vertex's position.
+template <int kBatch>
+__global__ void __launch_bounds__(256) k_rep_record(PassArgs A)
+{
+    const MeshView& m = A.m;
+    const EParams& p = A.p;
+    const PairCache& pc = A.pc;
+    __shared__ double s_rep[256 / kGroup];
+    __shared__ int s_cnt[256 / kGroup];
+    const int lb = logical_block();
+    const int t = lb * 256 + threadIdx.x;
+    const int v = t / kGroup, lane = t % kGroup;
+    const bool active = v < m.nv;
+    const int meta = active ? m.meta[v] : 0;
+    const bool ring = active && !is_generic(meta);
+    double rep = 0;
+    int cnt = 0;
+    if (active) {
+        const d3 xv = load_p(m.x, v);
+        if (lane == 0) {
+            pc.x_snap[3 * v] = xv.x;
+            pc.x_snap[3 * v + 1] = xv.y;
+            pc.x_snap[3 * v + 2] = xv.z;
+            if (!ring) pc.cell[v] = -1;
+        }
+        if (ring)
+            record_neighbours<kBatch>(m, A.nl, pc, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
+                rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
+                cnt++;
+            });
+    }
+    rep = group_sum(rep);
+    cnt = group_sum(cnt);
+    if (lane == 0) {
+        s_rep[threadIdx.x / kGroup] = rep;
+        s_cnt[threadIdx.x / kGroup] = cnt;
+    }
+    __syncthreads();
+    const int v0 = lb * (256 / kGroup);
+    const int i = threadIdx.x;
+    if (i < 256 / kGroup && v0 + i < m.nv) {
+        A.rep_v[v0 + i] = s_rep[i];
+        A.pair_cnt[v0 + i] = s_cnt[i];
+    }
+    // column 5 of pass-1 block lb (the same 32 vertices), as pass 1 sums it
+    __shared__ double s_red[4 * 6];
+    __shared__ double s_out[6];
+    double pr[6] = {0, 0, 0, 0, 0, 0};
+    if (active && lane == 0) {
+        pr[5] = ring ? rep : A.gprops[6 * v + 5];
+        if (A.vprops) A.vprops[6 * v + 5] = pr[5];
+    }
+    block_sum6_leaders(pr, s_red, s_out);
+    if (threadIdx.x == 5) A.partials[5 * A.nblocks + lb] = s_out[5];
 }
 
 // Pass 1 and the repulsion query in one launch of 2 x nblocks blocks: logical
@@ -792,9 +1079,10 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
         // repulsion (mesh_repulsion.h:56-79, doubled as in
         // mesh_properties.cpp:138-146) for the vertices k_rep found pairs for
         if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0) {
-            group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, [&](int, d3 d) {
-                acc += (p.kappa_r * 2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d;
-            });
+            // the lists k_rep recorded or checked at these positions
+            auto add = [&](int, d3 d) { acc += (p.kappa_r * 2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d; };
+            if (!(A.pc.x_snap && cached_neighbours(m, nl, A.pc, v, xv, p.rep_lc1, lane, add)))
+                group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, add);
         }
     }
     acc.x = group_sum(acc.x);
@@ -858,6 +1146,7 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
     a.list = b.list;
     a.nlist = b.nlist;
     a.nblocks = pass1_blocks(m.nv);
+    a.pc = a.rep_v ? b.pc : PairCache{};
     return a;
 }
 
@@ -866,7 +1155,7 @@ bool tuning_valid(const Tuning& t)
     auto th = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512; };
     return th(t.rep_threads) && th(t.p2_threads) &&
            (t.rep_batch == 1 || t.rep_batch == 2 || t.rep_batch == 4 || t.rep_batch == 8) &&
-           (t.rep_fused == 0 || t.rep_fused == 1);
+           (t.rep_fused == 0 || t.rep_fused == 1) && (t.pair_cache == 0 || t.pair_cache == 1);
 }
 
 int tuned_blocks(int nv, int threads) { return nblocks(nblocks(nv * kGroup, threads), kXcds) * kXcds; }
@@ -939,6 +1228,16 @@ static void pass1_rep_go(int batch, const PassArgs& a, hipStream_t s)
 void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
 {
     if (a.m.nv == 0) return;
+    if (a.rep_v && a.pc.x_snap) {
+        // the cached query writes the repulsion column of the partials itself
+        // (a second stream for it, beside pass 1, measured slower: the
+        // cross-queue fork and join cost more than the overlap gains)
+        launch_rep(a, t, s);
+        const dim3 grid(a.nblocks), block(256);
+        if (mixed) hipLaunchKernelGGL(k_pass1_apart<float>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(k_pass1_apart<double>, grid, block, 0, s, a);
+        return;
+    }
     if (a.rep_v && t.rep_fused) {
         if (mixed) pass1_rep_go<float>(t.rep_batch, a, s);
         else pass1_rep_go<double>(t.rep_batch, a, s);
@@ -952,7 +1251,30 @@ void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_
 
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s)
 {
-    if (a.rep_v && a.m.nv > 0) rep_dispatch(t, false, a, nullptr, s);
+    if (!a.rep_v || a.m.nv == 0) return;
+    if (!a.pc.x_snap) {
+        rep_dispatch(t, false, a, nullptr, s);
+        return;
+    }
+    const dim3 block(256);
+    if (a.pc.force) {
+        const dim3 grid(tuned_blocks(a.m.nv, 256));
+        switch (t.rep_batch) {
+        case 1: hipLaunchKernelGGL(k_rep_record<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_rep_record<2>, grid, block, 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_rep_record<8>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(k_rep_record<4>, grid, block, 0, s, a); break;
+        }
+        return;
+    }
+    hipLaunchKernelGGL(k_disp, dim3(kDispBlocks), dim3(256), 0, s, a.m.x, a.pc.x_snap, a.m.nv, a.pc.disp_part);
+    const dim3 grid(nblocks(nblocks(a.m.nv, kVertsPerBlock), kXcds) * kXcds);
+    switch (t.rep_batch) {
+    case 1: hipLaunchKernelGGL(k_rep_cached<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_rep_cached<2>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_rep_cached<8>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_rep_cached<4>, grid, block, 0, s, a); break;
+    }
 }
 
 void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)

@@ -5,6 +5,34 @@
 
 namespace tmh {
 
+// Pair cache of the cell-list repulsion query (single meshes).  Between
+// rebuilds of the cell list the candidates of a vertex are fixed (the stored
+// stencil list of its cell, or the stencil rows of the cell it has moved
+// to), and exclusion depends on the topology only.  A recording launch runs
+// the full query with radius lc1 + skin and keeps, per vertex and lane, the
+// non-excluded candidates it meets, in the lane's visit order; later
+// launches replay those lists with the exact lc1 test while every vertex has
+// moved less than skin / 2 since the recording (no other candidate can have
+// come within lc1) and the vertex is in the cell it was recorded in (the
+// full query would visit the same candidates in the same order).  Each lane
+// then adds the same terms in the same order as the full query: the results
+// are bit-identical.
+constexpr int kPairLanes = 8;      // lanes per vertex of the query
+constexpr int kPairCap = 4;        // recorded candidates per lane (more: that vertex takes the full query)
+constexpr int kDispBlocks = 128;   // blocks of k_disp, one partial each
+constexpr double kPairSkin = 0.2;  // skin = 0.2 lc1
+constexpr double kPairLimit = 0.45;    // replay while every vertex has moved < 0.45 skin (< skin / 2)
+struct PairCache {
+    double* x_snap;         // [3N] positions at the recording; null: no cache
+    int* cell;              // [N] cell id at the recording, -1: replay not allowed
+    unsigned* cnt;          // [N] recorded candidates of each lane, 4 bits per lane
+    int* ent;               // [8 kPairCap N] ids, lane-major per vertex
+    double* disp_part;      // [kDispBlocks] max |x - x_snap|^2 per block of k_disp
+    double skin;
+    double limit2;          // replay while max |x - x_snap|^2 < limit2
+    int force;              // 1: record in this launch
+};
+
 // energy passes (tm_energy.hip)
 struct PassBuffers {
     double* q;          // [N] curvature / area per vertex
@@ -16,6 +44,7 @@ struct PassBuffers {
     Globals* globals;
     const int* list;    // boundary / degenerate vertices
     int nlist;
+    PairCache pc;       // x_snap null: the query runs in full every time
 };
 int pass1_blocks(int nv);
 
@@ -39,6 +68,7 @@ struct PassArgs {
     const int* list;     // boundary / degenerate vertices
     int nlist;
     int nblocks;         // pass-1 blocks (a multiple of 8)
+    PairCache pc;
 };
 PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
                    double* grad);
@@ -52,6 +82,7 @@ struct Tuning {
     int rep_batch = 4;        // candidates in flight per lane: 1, 2, 4 or 8
     int p2_threads = 256;     // 64, 128, 256 or 512
     int rep_fused = 0;        // 1: the query shares the pass-1 launch (256 threads), 0: its own launch
+    int pair_cache = 1;       // 1: replay recorded candidate lists (single meshes, cell list)
 };
 bool tuning_valid(const Tuning& t);
 int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
