@@ -550,6 +550,22 @@ void launch_excl_mask(const MeshView& m, const NListView& nl, unsigned long long
 // n_k = e_k x e_{k-1} (|n_k| = 2 area(F_k)) and that of j_k -> v is n_{k+1}.
 // One slot contributes area(F_k), vol(F_k) = x_v . n_k / 6, l_k theta_k / 2
 // and the tether term of halfedge v -> j_k (mesh_properties.cpp:19-33).
+// Slot k's terms from its spoke e_k = x_{j_k} - x_v and the sector normals
+// n_k = e_k x e_{k-1} and n_{k+1}.
+template <class T>
+__device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e_cur, d3 n_cur, d3 n_next,
+                                                   double& area, double& vol, double& curv, double& teth)
+{
+    T af = Prec<T>::sqrt_(sqnorm(n_cur)) * (T)0.5;
+    T l = Prec<T>::sqrt_(sqnorm(e_cur));
+    T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
+    curv += (double)((T)0.5 * l * th);
+    area += (double)af;
+    vol += dot(xv, n_cur) / 6;    // = x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
+    if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
+    else teth += bond_area_value((double)af, p);
+}
+
 template <class T>
 __device__ __forceinline__ void slot_properties(const MeshView& m, const EParams& p, int v, int n, int k, d3 xv,
                                                 double& area, double& vol, double& curv, double& teth)
@@ -561,14 +577,26 @@ __device__ __forceinline__ void slot_properties(const MeshView& m, const EParams
     const d3 e_next = load_p(m.x, m.ring[kp * N + v]) - xv;
     const d3 n_cur = cross(e_cur, e_prev);
     const d3 n_next = cross(e_next, e_cur);
-    T af = Prec<T>::sqrt_(sqnorm(n_cur)) * (T)0.5;
-    T l = Prec<T>::sqrt_(sqnorm(e_cur));
-    T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
-    curv += (double)((T)0.5 * l * th);
-    area += (double)af;
-    vol += dot(xv, n_cur) / 6;    // = x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
-    if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
-    else teth += bond_area_value((double)af, p);
+    slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
+}
+
+__device__ __forceinline__ d3 shfl_d3(d3 a, int src)
+{
+    return d3{__shfl(a.x, src, 64), __shfl(a.y, src, 64), __shfl(a.z, src, 64)};
+}
+
+// Rings of at most kGroup: lane k of the group owns slot k and loads only
+// its own neighbour; e_{k-1} comes from lane k-1 and n_{k+1} from lane k+1
+// (the values those lanes computed, so the bits are those of
+// slot_properties).  Lanes n..kGroup-1 mirror slot 0 and add nothing.
+struct RingLanes {
+    int k, src_prev, src_next;
+};
+__device__ __forceinline__ RingLanes ring_lanes(int n, int lane)
+{
+    const int k = lane < n ? lane : 0;
+    const int gl = (int)(threadIdx.x & 63) - lane;    // first lane of the group in the wave
+    return RingLanes{k, gl + (k == 0 ? n - 1 : k - 1), gl + (k + 1 == n ? 0 : k + 1)};
 }
 
 __device__ __forceinline__ bool is_generic(int meta)
@@ -612,7 +640,16 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     double area = 0, vol = 0, curv = 0, teth = 0;
     if (ring) {
         const d3 xv = load_p(m.x, v);
-        for (int k = lane; k < n; k += kGroup) slot_properties<T>(m, p, v, n, k, xv, area, vol, curv, teth);
+        if (n <= kGroup) {
+            const RingLanes r = ring_lanes(n, lane);
+            const d3 e_cur = load_p(m.x, m.ring[r.k * m.nv + v]) - xv;
+            const d3 e_prev = shfl_d3(e_cur, r.src_prev);
+            const d3 n_cur = cross(e_cur, e_prev);
+            const d3 n_next = shfl_d3(n_cur, r.src_next);
+            if (lane < n) slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
+        } else {
+            for (int k = lane; k < n; k += kGroup) slot_properties<T>(m, p, v, n, k, xv, area, vol, curv, teth);
+        }
     }
     area = group_sum(area);
     vol = group_sum(vol);
@@ -1021,19 +1058,10 @@ __global__ void __launch_bounds__(128) k_generic_grad_at(const PassArgs* __restr
 // The penalty factors are known when pass 2 runs, so every contribution goes
 // straight into one accumulator.
 template <class T>
-__device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p, const Globals& g,
-                                            const double* __restrict__ q, int v, int n, int k, d3 xv, double qv)
+__device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& g, double qv, d3 e_prev, d3 e_cur,
+                                               double q_prev, double q_cur, d3 n_cur, d3 n_next)
 {
-    const int N = m.nv;
-    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
-    const int jm = m.ring[km * N + v], jc = m.ring[k * N + v], jp = m.ring[kp * N + v];
-    const d3 e_prev = load_p(m.x, jm) - xv;
-    const d3 e_cur = load_p(m.x, jc) - xv;
-    const d3 e_next = load_p(m.x, jp) - xv;
-    const double q_prev = q[jm], q_cur = q[jc];
     const double kb = p.kappa_b, kt = p.kappa_t;
-    const d3 n_cur = cross(e_cur, e_prev);
-    const d3 n_next = cross(e_next, e_cur);
     // face F_k = (v, j_k, j_{k-1})
     T n0l = Prec<T>::sqrt_(sqnorm(n_cur));
     T inv0 = (T)1 / n0l;
@@ -1058,6 +1086,22 @@ __device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p,
     return acc;
 }
 
+template <class T>
+__device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p, const Globals& g,
+                                            const double* __restrict__ q, int v, int n, int k, d3 xv, double qv)
+{
+    const int N = m.nv;
+    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
+    const int jm = m.ring[km * N + v], jc = m.ring[k * N + v], jp = m.ring[kp * N + v];
+    const d3 e_prev = load_p(m.x, jm) - xv;
+    const d3 e_cur = load_p(m.x, jc) - xv;
+    const d3 e_next = load_p(m.x, jp) - xv;
+    const double q_prev = q[jm], q_cur = q[jc];
+    const d3 n_cur = cross(e_cur, e_prev);
+    const d3 n_next = cross(e_next, e_cur);
+    return slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
+}
+
 template <class T, int kThreads>
 __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
@@ -1075,7 +1119,20 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
         const Globals g = *A.globals;
         const d3 xv = load_p(m.x, v);
         const double qv = A.q[v];
-        for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, A.q, v, n, k, xv, qv);
+        if (n <= kGroup) {
+            // as in pass 1: e_{k-1}, q_{k-1} from lane k-1, n_{k+1} from lane k+1
+            const RingLanes r = ring_lanes(n, lane);
+            const int jc = m.ring[r.k * m.nv + v];
+            const d3 e_cur = load_p(m.x, jc) - xv;
+            const double q_cur = A.q[jc];
+            const d3 e_prev = shfl_d3(e_cur, r.src_prev);
+            const double q_prev = __shfl(q_cur, r.src_prev, 64);
+            const d3 n_cur = cross(e_cur, e_prev);
+            const d3 n_next = shfl_d3(n_cur, r.src_next);
+            if (lane < n) acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
+        } else {
+            for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, A.q, v, n, k, xv, qv);
+        }
         // repulsion (mesh_repulsion.h:56-79, doubled as in
         // mesh_properties.cpp:138-146) for the vertices k_rep found pairs for
         if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0) {
