@@ -67,7 +67,7 @@ struct Globals {
     double props[6];    // area, volume, curvature, bending, tethering, repulsion
     double energy;
     double fac_a, fac_v, fac_c;    // dE/dA, dE/dV, dE/dC (kernel.h:27-71)
-    double pad;
+    double fac_v6;                 // fac_v / 6 (the volume gradient's factor per sector normal)
 };
 
 // Read-only device view of a mesh.

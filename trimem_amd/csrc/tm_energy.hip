@@ -147,6 +147,7 @@ __device__ void finalize_globals(const double* __restrict__ partials, int nblock
         d = V / p.ref_volume - 1.0;
         e += p.kappa_v * d * d;
         g->fac_v = 2.0 * p.kappa_v / p.ref_volume * d;
+        g->fac_v6 = g->fac_v / 6.0;
         d = C / p.ref_curvature - 1.0;
         e += p.kappa_c * d * d;
         g->fac_c = 2.0 * p.kappa_c / p.ref_curvature * d;
@@ -561,7 +562,7 @@ __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     curv += (double)((T)0.5 * l * th);
     area += (double)af;
-    vol += dot(xv, n_cur) / 6;    // = x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
+    vol += dot(xv, n_cur);    // 6 x the signed volume x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
     if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
     else teth += bond_area_value((double)af, p);
 }
@@ -658,19 +659,23 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     double pr[6] = {0, 0, 0, 0, 0, 0};
     __shared__ double s_q[256 / kGroup];
     if (active && lane == 0) {
+        double q;    // curvature / area
         if (ring) {
-            double a = area / 3, c = curv / 2;
+            // a third of the face areas and volumes, half the edge terms
+            const double a = area / 3, c = curv * 0.5;
+            q = c / a;
             pr[0] = a;
-            pr[1] = vol / 3;
+            pr[1] = vol / 18;
             pr[2] = c;
-            pr[3] = 2 * c * c / a;
-            pr[4] = teth / (p.bond_type == 0 ? 2 : 3);
+            pr[3] = 2 * c * q;
+            pr[4] = p.bond_type == 0 ? teth * 0.5 : teth / 3;
             pr[5] = rv;
         } else {
             for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
+            q = pr[2] / pr[0];
         }
         if (kRepApart) pr[5] = 0.0;
-        s_q[threadIdx.x / kGroup] = pr[2] / pr[0];
+        s_q[threadIdx.x / kGroup] = q;
         if (A.vprops) {
             for (int k = 0; k < (kRepApart ? 5 : 6); k++) A.vprops[6 * v + k] = pr[k];
         }
@@ -1075,7 +1080,7 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     double fdA = g.fac_a - kb * (2.0 / 3.0) * q2;
     if (p.bond_type == 1) fdA += kt * bond_area_dfac((double)(n0l * (T)0.5), p);
     d3 acc = fdA * dA;
-    acc += (g.fac_v / 6.0) * n_cur;
+    acc += g.fac_v6 * n_cur;
     acc += (kb * wq) * u;
     // spoke (v, j_k) with faces F_k and F_{k+1}
     T l = Prec<T>::sqrt_(sqnorm(e_cur));
