@@ -42,6 +42,7 @@ struct Prec;
 template <>
 struct Prec<double> {
     __device__ static __forceinline__ double sqrt_(double a) { return sqrt(a); }
+    __device__ static __forceinline__ double rcp(double a) { return 1.0 / a; }
     __device__ static __forceinline__ double dihedral(const d3& n0, const d3& n1, const d3& e, double)
     {
         // OpenMesh calc_dihedral_angle: acos of the normalised dot, signed
@@ -53,15 +54,18 @@ struct Prec<double> {
 template <>
 struct Prec<float> {
     __device__ static __forceinline__ float sqrt_(double a) { return sqrtf((float)a); }
+    // the hardware reciprocal (1 ulp) instead of the correctly rounded division
+    __device__ static __forceinline__ float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
     __device__ static __forceinline__ float dihedral(const d3& n0, const d3& n1, const d3& e, float l)
     {
         // the same angle as atan2(|n0 x n1|, n0.n1), signed by (n0 x n1).e;
-        // n0 x n1 is parallel to e, so (n0 x n1).e / |e| is the signed sine
-        // term.  Both arguments are formed in FP64 (they cancel for small
-        // angles) and rounded once.
+        // n0 x n1 is parallel to e, so (n0 x n1).e = |e| x the signed sine
+        // term, and atan2 of (sine |e|, cosine |e|) is the angle.  Both
+        // products are formed in FP64 (they cancel for small angles) and
+        // rounded once.
         double s = dot(cross(n0, n1), e);
-        double c = dot(n0, n1);
-        return atan2f((float)s / l, (float)c);
+        double c = dot(n0, n1) * (double)l;
+        return atan2f((float)s, (float)c);
     }
 };
 
@@ -1069,7 +1073,7 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     const double kb = p.kappa_b, kt = p.kappa_t;
     // face F_k = (v, j_k, j_{k-1})
     T n0l = Prec<T>::sqrt_(sqnorm(n_cur));
-    T inv0 = (T)1 / n0l;
+    T inv0 = Prec<T>::rcp(n0l);
     d3 rim = e_prev - e_cur;                               // x_{k-1} - x_k
     T cot_j = (T)(-dot(rim, e_cur)) * inv0;                // angle at j_k
     T cot_i = (T)dot(e_prev, rim) * inv0;                  // angle at j_{k-1}
@@ -1087,7 +1091,7 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     double fe = -(kb * (qv + q_cur) + 0.5 * g.fac_c) * (double)th;
     if (p.bond_type == 0) fe -= kt * bond_edge_dfac((double)l, p);
-    acc += (fe * (double)((T)1 / l)) * e_cur;
+    acc += (fe * (double)Prec<T>::rcp(l)) * e_cur;
     return acc;
 }
 
