@@ -4,8 +4,9 @@ module trimem_amd.core in-tree.
     python -m trimem_amd.build          # or __graft_entry__.build()
 
 hipcc compiles every .hip file for gfx950 only.  -ffp-contract=off keeps the
-neighbour-list distance tests and the FP64 path IEEE-exact (no fused a*b+c),
-and packed FP32 instructions are disabled.
+neighbour-list distance tests IEEE-exact like the reference's (no fused
+a*b+c; they decide which pairs exist); the ring passes' slot terms opt in to
+contraction locally.  Packed FP32 instructions are disabled.
 """
 import os
 import subprocess

@@ -34,6 +34,26 @@
 
 namespace tmh {
 
+// FMA-contracted vector arithmetic for the ring passes' slot terms, whose
+// parity is a tolerance.  The neighbour-list distance tests do not use these:
+// their rounding decides which pairs exist and stays uncontracted, like the
+// reference's.
+__device__ __forceinline__ d3 cross_f(const d3& a, const d3& b)
+{
+#pragma clang fp contract(fast)
+    return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dot_f(const d3& a, const d3& b)
+{
+#pragma clang fp contract(fast)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ d3 axpy_f(double s, const d3& a, const d3& y)    // s a + y
+{
+#pragma clang fp contract(fast)
+    return d3{s * a.x + y.x, s * a.y + y.y, s * a.z + y.z};
+}
+
 // Scalar kernels of the two precision modes.  Geometry vectors (edge vectors,
 // face normals and the inputs of every angle) are always FP64; T is the type
 // of the square roots, reciprocals, cotangents and angles built from them.
@@ -46,9 +66,9 @@ struct Prec<double> {
     __device__ static __forceinline__ double dihedral(const d3& n0, const d3& n1, const d3& e, double)
     {
         // OpenMesh calc_dihedral_angle: acos of the normalised dot, signed
-        double denom = sqrt(sqnorm(n0)) * sqrt(sqnorm(n1));
+        double denom = sqrt(dot_f(n0, n0)) * sqrt(dot_f(n1, n1));
         if (denom == 0) return 0;
-        return om_angle(dot(n0, n1) / denom, dot(cross(n0, n1), e));
+        return om_angle(dot_f(n0, n1) / denom, dot_f(cross_f(n0, n1), e));
     }
 };
 template <>
@@ -63,8 +83,8 @@ struct Prec<float> {
         // term, and atan2 of (sine |e|, cosine |e|) is the angle.  Both
         // products are formed in FP64 (they cancel for small angles) and
         // rounded once.
-        double s = dot(cross(n0, n1), e);
-        double c = dot(n0, n1) * (double)l;
+        double s = dot_f(cross_f(n0, n1), e);
+        double c = dot_f(n0, n1) * (double)l;
         return atan2f((float)s, (float)c);
     }
 };
@@ -561,12 +581,12 @@ template <class T>
 __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e_cur, d3 n_cur, d3 n_next,
                                                    double& area, double& vol, double& curv, double& teth)
 {
-    T af = Prec<T>::sqrt_(sqnorm(n_cur)) * (T)0.5;
-    T l = Prec<T>::sqrt_(sqnorm(e_cur));
+    T af = Prec<T>::sqrt_(dot_f(n_cur, n_cur)) * (T)0.5;
+    T l = Prec<T>::sqrt_(dot_f(e_cur, e_cur));
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     curv += (double)((T)0.5 * l * th);
     area += (double)af;
-    vol += dot(xv, n_cur);    // 6 x the signed volume x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
+    vol += dot_f(xv, n_cur);    // 6 x the signed volume x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
     if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
     else teth += bond_area_value((double)af, p);
 }
@@ -580,8 +600,8 @@ __device__ __forceinline__ void slot_properties(const MeshView& m, const EParams
     const d3 e_prev = load_p(m.x, m.ring[km * N + v]) - xv;
     const d3 e_cur = load_p(m.x, m.ring[k * N + v]) - xv;
     const d3 e_next = load_p(m.x, m.ring[kp * N + v]) - xv;
-    const d3 n_cur = cross(e_cur, e_prev);
-    const d3 n_next = cross(e_next, e_cur);
+    const d3 n_cur = cross_f(e_cur, e_prev);
+    const d3 n_next = cross_f(e_next, e_cur);
     slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
 }
 
@@ -649,7 +669,7 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
             const RingLanes r = ring_lanes(n, lane);
             const d3 e_cur = load_p(m.x, m.ring[r.k * m.nv + v]) - xv;
             const d3 e_prev = shfl_d3(e_cur, r.src_prev);
-            const d3 n_cur = cross(e_cur, e_prev);
+            const d3 n_cur = cross_f(e_cur, e_prev);
             const d3 n_next = shfl_d3(n_cur, r.src_next);
             if (lane < n) slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
         } else {
@@ -1070,29 +1090,29 @@ template <class T>
 __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& g, double qv, d3 e_prev, d3 e_cur,
                                                double q_prev, double q_cur, d3 n_cur, d3 n_next)
 {
+#pragma clang fp contract(fast)
     const double kb = p.kappa_b, kt = p.kappa_t;
     // face F_k = (v, j_k, j_{k-1})
-    T n0l = Prec<T>::sqrt_(sqnorm(n_cur));
+    T n0l = Prec<T>::sqrt_(dot_f(n_cur, n_cur));
     T inv0 = Prec<T>::rcp(n0l);
     d3 rim = e_prev - e_cur;                               // x_{k-1} - x_k
-    T cot_j = (T)(-dot(rim, e_cur)) * inv0;                // angle at j_k
-    T cot_i = (T)dot(e_prev, rim) * inv0;                  // angle at j_{k-1}
+    T cot_j = (T)(-dot_f(rim, e_cur)) * inv0;              // angle at j_k
+    T cot_i = (T)dot_f(e_prev, rim) * inv0;                // angle at j_{k-1}
     d3 u = (double)inv0 * n_cur;
-    d3 dA = 0.5 * cross(u, rim);                           // d area(F_k) / d x_v
+    d3 dA = 0.5 * cross_f(u, rim);                         // d area(F_k) / d x_v
     double wq = (double)cot_j * (qv - q_prev) + (double)cot_i * (qv - q_cur);
     double q2 = qv * qv + q_cur * q_cur + q_prev * q_prev;
     double fdA = g.fac_a - kb * (2.0 / 3.0) * q2;
     if (p.bond_type == 1) fdA += kt * bond_area_dfac((double)(n0l * (T)0.5), p);
     d3 acc = fdA * dA;
-    acc += g.fac_v6 * n_cur;
-    acc += (kb * wq) * u;
+    acc = axpy_f(g.fac_v6, n_cur, acc);
+    acc = axpy_f(kb * wq, u, acc);
     // spoke (v, j_k) with faces F_k and F_{k+1}
-    T l = Prec<T>::sqrt_(sqnorm(e_cur));
+    T l = Prec<T>::sqrt_(dot_f(e_cur, e_cur));
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     double fe = -(kb * (qv + q_cur) + 0.5 * g.fac_c) * (double)th;
     if (p.bond_type == 0) fe -= kt * bond_edge_dfac((double)l, p);
-    acc += (fe * (double)Prec<T>::rcp(l)) * e_cur;
-    return acc;
+    return axpy_f(fe * (double)Prec<T>::rcp(l), e_cur, acc);
 }
 
 template <class T>
@@ -1106,8 +1126,8 @@ __device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p,
     const d3 e_cur = load_p(m.x, jc) - xv;
     const d3 e_next = load_p(m.x, jp) - xv;
     const double q_prev = q[jm], q_cur = q[jc];
-    const d3 n_cur = cross(e_cur, e_prev);
-    const d3 n_next = cross(e_next, e_cur);
+    const d3 n_cur = cross_f(e_cur, e_prev);
+    const d3 n_next = cross_f(e_next, e_cur);
     return slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
 }
 
@@ -1136,7 +1156,7 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
             const double q_cur = A.q[jc];
             const d3 e_prev = shfl_d3(e_cur, r.src_prev);
             const double q_prev = __shfl(q_cur, r.src_prev, 64);
-            const d3 n_cur = cross(e_cur, e_prev);
+            const d3 n_cur = cross_f(e_cur, e_prev);
             const d3 n_next = shfl_d3(n_cur, r.src_next);
             if (lane < n) acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
         } else {
