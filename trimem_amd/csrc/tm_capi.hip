@@ -576,7 +576,7 @@ struct tm_energy {
     int nv = 0;
     DevBuf<double> q, partials, grad, vprops, gprops;
     DevBuf<int> pair_cnt;
-    DevBuf<double> rep_v;
+    DevBuf<double> rep_v, rep_g;
     DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<int> flip_result, proposals, pf_status;
@@ -614,6 +614,7 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     E.pair_cnt.alloc(M.nv);
     HIPCHECK(hipMemsetAsync(E.pair_cnt.p, 0, sizeof(int) * M.nv, g_rt.stream));
     E.rep_v.alloc(M.nv);
+    E.rep_g.alloc((size_t)M.nv * 3);
     E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
     E.grad.alloc((size_t)M.nv * 3);
     E.counter.alloc(2);
@@ -629,6 +630,7 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.gprops = E.gprops.p;
     b.pair_cnt = E.pair_cnt.p;
     b.rep_v = E.rep_v.p;
+    b.rep_g = E.rep_g.p;
     b.partials = E.partials.p;
     b.counter = E.counter.p;
     b.globals = E.globals.p;

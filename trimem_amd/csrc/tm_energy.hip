@@ -422,6 +422,42 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
     }
 }
 
+// ---------------------------------------------------- repulsion terms ---
+// One lane's running repulsion terms over the pairs it visits: the property
+// sum(rep(l)) (mesh_repulsion.h:28-54), the pair count and the gradient at
+// v, 2 kappa_r sum(dfac(l) d) (mesh_repulsion.h:56-79, doubled as in
+// mesh_properties.cpp:138-146).  The query kernels sum them over the lanes
+// with group_sum; pass 2 adds the gradient of the vertices with pairs.
+struct RepTerms {
+    double rep = 0.0;
+    d3 g{0.0, 0.0, 0.0};
+    int cnt = 0;
+    __device__ __forceinline__ void add(d3 d, const EParams& p)
+    {
+        const double l = norm(d);
+        rep += rep_value(l, p.rep_lc1, p.rep_r);
+        g += (p.kappa_r * 2.0 * rep_dfac(l, p.rep_lc1, p.rep_r)) * d;
+        cnt++;
+    }
+    __device__ __forceinline__ void group_reduce()
+    {
+        rep = group_sum(rep);
+        g.x = group_sum(g.x);
+        g.y = group_sum(g.y);
+        g.z = group_sum(g.z);
+        cnt = group_sum(cnt);
+    }
+};
+// the gradient row of a vertex with pairs (pass 2 reads it only then)
+__device__ __forceinline__ void store_rep_g(const PassArgs& A, int v, const RepTerms& t)
+{
+    if (A.rep_g && t.cnt > 0) {
+        A.rep_g[3 * v] = t.g.x;
+        A.rep_g[3 * v + 1] = t.g.y;
+        A.rep_g[3 * v + 2] = t.g.z;
+    }
+}
+
 // ------------------------------------------------------------ pair cache ---
 // (PairCache in tm_internal.h.)  Cell of v at position pi, or -1 outside the
 // grid.
@@ -761,20 +797,18 @@ __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
     const bool ring = active && !is_generic(meta);
-    double rep = 0;
-    int cnt = 0;
+    RepTerms tr;
     if (ring) {
         const d3 xv = load_p(m.x, v);
-        group_neighbours<kBatch>(m, A.nl, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
-            rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
-            cnt++;
-        });
+        group_neighbours<kBatch>(m, A.nl, v, meta_valence(meta), xv, p.rep_lc1, lane,
+                                 [&](int, d3 d) { tr.add(d, p); });
     }
-    rep = group_sum(rep);
-    cnt = group_sum(cnt);
+    tr.group_reduce();
+    const double rep = tr.rep;
     if (lane == 0) {
         s_rep[threadIdx.x / kGroup] = rep;
-        s_cnt[threadIdx.x / kGroup] = cnt;
+        s_cnt[threadIdx.x / kGroup] = tr.cnt;
+        if (ring) store_rep_g(A, v, tr);
     }
     __syncthreads();
     const int v0 = (lb * kThreads) / kGroup;
@@ -844,25 +878,38 @@ __global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
         } else if (ring) {
             const int cell = pc.cell[v];
             const unsigned word = pc.cnt[v];
-            if (cell >= 0 && query_cell(A.nl, xv) == cell) {
+            const bool replay = cell >= 0 && query_cell(A.nl, xv) == cell;
+            if (replay && word == 0u) {
+                done = true;    // nothing recorded: no pairs (the sums below would all be +0)
+            } else if (replay) {
                 const double dmax = p.rep_lc1, dmax2 = dmax * dmax * (1.0 + 1e-15);
-                double ls[kGroup];
-#pragma unroll
-                for (int l = 0; l < kGroup; l++) {
+                // lane l's terms, then lane pairs, as group_sum adds them
+                auto lane_terms = [&](int l) {
+                    RepTerms t;
                     const int k = (int)((word >> (4 * l)) & 15u);
                     const int* e = pc.ent + ((size_t)v * kGroup + l) * kPairCap;
-                    double acc = 0.0;
                     for (int i = 0; i < k; i++) {
                         const d3 d = xv - load_p(m.x, e[i]);
                         const double sq = sqnorm(d);
-                        if (sq <= dmax2 && sqrt(sq) < dmax) {
-                            acc += rep_value(norm(d), p.rep_lc1, p.rep_r);
-                            cnt++;
-                        }
+                        if (sq <= dmax2 && sqrt(sq) < dmax) t.add(d, p);
                     }
-                    ls[l] = acc;
+                    return t;
+                };
+                double rh[kGroup / 2];
+                d3 gh[kGroup / 2];
+#pragma unroll
+                for (int h = 0; h < kGroup / 2; h++) {
+                    const RepTerms a = lane_terms(2 * h), b = lane_terms(2 * h + 1);
+                    rh[h] = a.rep + b.rep;
+                    gh[h] = a.g + b.g;
+                    cnt += a.cnt + b.cnt;
                 }
-                rep = ((ls[0] + ls[1]) + (ls[2] + ls[3])) + ((ls[4] + ls[5]) + (ls[6] + ls[7]));
+                RepTerms t;
+                t.rep = (rh[0] + rh[1]) + (rh[2] + rh[3]);
+                t.g = (gh[0] + gh[1]) + (gh[2] + gh[3]);
+                t.cnt = cnt;
+                rep = t.rep;
+                store_rep_g(A, v, t);
                 done = true;
             }
         }
@@ -881,20 +928,16 @@ __global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
         const int u = s_list[i];
         const d3 xu = load_p(m.x, u);
         const int n = meta_valence(m.meta[u]);
-        double rep = 0.0;
-        int cnt = 0;
-        auto add = [&](int, d3 d) {
-            rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
-            cnt++;
-        };
+        RepTerms tr;
+        auto add = [&](int, d3 d) { tr.add(d, p); };
         if (mode == 2) record_neighbours<kBatch>(m, A.nl, pc, u, n, xu, p.rep_lc1, lane, add);
         else group_neighbours<kBatch>(m, A.nl, u, n, xu, p.rep_lc1, lane, add);
-        rep = group_sum(rep);
-        cnt = group_sum(cnt);
+        tr.group_reduce();
         if (lane == 0) {
-            A.rep_v[u] = rep;
-            A.pair_cnt[u] = cnt;
-            s_val[u - base] = rep;
+            A.rep_v[u] = tr.rep;
+            A.pair_cnt[u] = tr.cnt;
+            s_val[u - base] = tr.rep;
+            store_rep_g(A, u, tr);
         }
     }
     __syncthreads();
@@ -934,8 +977,7 @@ __global__ void __launch_bounds__(256) k_rep_record(PassArgs A)
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
     const bool ring = active && !is_generic(meta);
-    double rep = 0;
-    int cnt = 0;
+    RepTerms tr;
     if (active) {
         const d3 xv = load_p(m.x, v);
         if (lane == 0) {
@@ -945,16 +987,15 @@ __global__ void __launch_bounds__(256) k_rep_record(PassArgs A)
             if (!ring) pc.cell[v] = -1;
         }
         if (ring)
-            record_neighbours<kBatch>(m, A.nl, pc, v, meta_valence(meta), xv, p.rep_lc1, lane, [&](int, d3 d) {
-                rep += rep_value(norm(d), p.rep_lc1, p.rep_r);
-                cnt++;
-            });
+            record_neighbours<kBatch>(m, A.nl, pc, v, meta_valence(meta), xv, p.rep_lc1, lane,
+                                      [&](int, d3 d) { tr.add(d, p); });
     }
-    rep = group_sum(rep);
-    cnt = group_sum(cnt);
+    tr.group_reduce();
+    const double rep = tr.rep;
     if (lane == 0) {
         s_rep[threadIdx.x / kGroup] = rep;
-        s_cnt[threadIdx.x / kGroup] = cnt;
+        s_cnt[threadIdx.x / kGroup] = tr.cnt;
+        if (ring) store_rep_g(A, v, tr);
     }
     __syncthreads();
     const int v0 = lb * (256 / kGroup);
@@ -1162,18 +1203,14 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
         } else {
             for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, A.q, v, n, k, xv, qv);
         }
-        // repulsion (mesh_repulsion.h:56-79, doubled as in
-        // mesh_properties.cpp:138-146) for the vertices k_rep found pairs for
-        if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0) {
-            // the lists k_rep recorded or checked at these positions
-            auto add = [&](int, d3 d) { acc += (p.kappa_r * 2.0 * rep_dfac(norm(d), p.rep_lc1, p.rep_r)) * d; };
-            if (!(A.pc.x_snap && cached_neighbours(m, nl, A.pc, v, xv, p.rep_lc1, lane, add)))
-                group_neighbours(m, nl, v, n, xv, p.rep_lc1, lane, add);
-        }
     }
     acc.x = group_sum(acc.x);
     acc.y = group_sum(acc.y);
     acc.z = group_sum(acc.z);
+    // repulsion gradient of the vertices the query found pairs for (written
+    // by the query kernel of this evaluation)
+    if (lane == 0 && ring && A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0)
+        acc += d3{A.rep_g[3 * v], A.rep_g[3 * v + 1], A.rep_g[3 * v + 2]};
     // stage the block's 32 gradient rows and store them as contiguous doubles;
     // rows of listed (boundary) vertices are left to k_generic_grad
     __shared__ double s_g[3 * kThreads / kGroup];
@@ -1225,6 +1262,7 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
     a.gprops = b.gprops;
     a.rep_v = (p.rep_lc1 > 0 && nl.kind >= 0) ? b.rep_v : nullptr;
     a.pair_cnt = b.pair_cnt;
+    a.rep_g = b.rep_g;
     a.partials = b.partials;
     a.globals = b.globals;
     a.grad = grad;

@@ -39,6 +39,7 @@ struct PassBuffers {
     double* gprops;     // [6N] properties of listed (boundary) vertices
     int* pair_cnt;      // [N] repulsion pairs per vertex (k_rep)
     double* rep_v;      // [N] repulsion property per vertex (k_rep)
+    double* rep_g;      // [3N] repulsion gradient, rows of vertices with pairs (k_rep)
     double* partials;   // [6 * pass1 blocks]
     unsigned* counter;  // last-block counter of pass 1 (returns to 0)
     Globals* globals;
@@ -61,6 +62,7 @@ struct PassArgs {
     double* gprops;
     double* rep_v;       // null: repulsion off
     int* pair_cnt;
+    double* rep_g;       // null: rows not stored (pass 1 only)
     double* partials;
     Globals* globals;
     double* grad;
@@ -79,7 +81,7 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
 // same 32 vertices as pass-1 blocks and sum their column the same way.)
 struct Tuning {
     int rep_threads = 256;    // 64, 128, 256 or 512
-    int rep_batch = 4;        // candidates in flight per lane: 1, 2, 4 or 8
+    int rep_batch = 2;        // candidates in flight per lane: 1, 2, 4 or 8
     int p2_threads = 256;     // 64, 128, 256 or 512
     int rep_fused = 0;        // 1: the query shares the pass-1 launch (256 threads), 0: its own launch
     int pair_cache = 1;       // 1: replay recorded candidate lists (single meshes, cell list)
