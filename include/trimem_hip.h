@@ -151,7 +151,9 @@ int tm_stream(void** out);
 /* make_nlist(mesh, eparams) */
 int tm_nlist_create(tm_mesh* mesh, const tm_params* params, tm_nlist** out);
 int tm_nlist_destroy(tm_nlist* nl);
-/* the energy manager's own list (EnergyManager::nlist), borrowed */
+/* the energy manager's own list (EnergyManager::nlist), borrowed: after
+ * tm_energy_update_repulsion it is the previous list (the next rebuild
+ * reuses its buffers), so ask again */
 int tm_energy_nlist(tm_energy* em, tm_nlist** out);
 /* distance_matrix(mesh, dmax): count first (d, i, j may be NULL), then fill */
 int tm_nlist_distance_matrix(tm_nlist* nl, tm_mesh* mesh, double dmax, double* d, int* i, int* j, long cap,

@@ -583,7 +583,12 @@ struct tm_energy {
     DevBuf<double> uniforms, pf_dprops, pf_state;
     DevBuf<unsigned long long> pf_owner;
     Tuning tune;
-    ~tm_energy() { delete nlist; }
+    tm_nlist* spare = nullptr;    // the previous list, rebuilt in place by update_repulsion
+    ~tm_energy()
+    {
+        delete nlist;
+        delete spare;
+    }
 };
 
 namespace {
@@ -621,6 +626,22 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     E.globals.alloc(1);
     E.flip_result.alloc(4);
     HIPCHECK(hipMemsetAsync(E.counter.p, 0, 2 * sizeof(unsigned), g_rt.stream));
+}
+
+// update_repulsion (energy.cpp:57-61) without new device allocations: the
+// list is built into the energy manager's spare, whose buffers remain from
+// the rebuild before, and swapped in only when complete, so a failed build
+// leaves the current list in place.
+void rebuild_nlist(tm_energy& E, tm_mesh& M)
+{
+    check_params(E.params);
+    if (!E.spare) E.spare = new tm_nlist();
+    tm_nlist& L = *E.spare;
+    L.kind = E.params.rep_nlist;
+    L.exclusion = E.params.rep_exclusion;
+    L.rlist = E.params.rep_rlist;
+    build_nlist(L, M);
+    std::swap(E.nlist, E.spare);
 }
 
 PassBuffers buffers(tm_energy& E, tm_mesh& M)
@@ -1006,9 +1027,7 @@ int tm_energy_update_repulsion(tm_energy* em, tm_mesh* mesh)
 {
     API_BEGIN
     upload_points(*mesh);
-    tm_nlist* fresh = make_nlist(*mesh, em->params);
-    delete em->nlist;
-    em->nlist = fresh;
+    rebuild_nlist(*em, *mesh);
     API_END
 }
 
@@ -1703,9 +1722,7 @@ int tm_replicas_update_repulsion(tm_replicas* rs)
         tm_mesh& M = *S.meshes[r];
         tm_energy& E = *S.ems[r];
         upload_points(M);
-        tm_nlist* fresh = make_nlist(M, E.params);
-        delete E.nlist;
-        E.nlist = fresh;
+        rebuild_nlist(E, M);
     }
     refresh_replica_args(S);
     API_END
