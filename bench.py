@@ -129,7 +129,8 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
     """HMC steps/s of the same chain two ways: the device-resident step
     (DeviceMeshHMC, device draws) and the drop-in path (MeshHMC over
     EnergyEvaluators over trimem_amd.core, numpy draws, positions through
-    the host view on every force call).  Both: dt 1e-5, 10 leapfrog steps,
+    the host view on every force call).  Both: dt 5e-6 (acceptance ~0.95 at
+    L7; at 1e-5 most trajectories diverge on the stiff tethers), 10 leapfrog steps,
     mass 1, T 1, neighbour list rebuilt after every step."""
     import contextlib
     import io
@@ -138,7 +139,7 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
     from trimem_amd.mc import hmc as H
     from trimem_amd.mc.mesh import Mesh
 
-    opts = {"time_step": 1e-5, "num_integration_steps": 10, "info_step": 0}
+    opts = {"time_step": 5e-6, "num_integration_steps": 10, "info_step": 0}
 
     def device_chain(pair_cache):
         mesh = Mesh(v, f)
@@ -183,7 +184,7 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
             "device_over_dropin": (dev_steps / t_dev) / (host_steps / t_host),
             "device_acceptance": acc / dev_steps,
             "device_steps": dev_steps, "dropin_steps": host_steps,
-            "trajectory": "dt 1e-5, 10 leapfrog steps (11 force evaluations), mass 1, T 1, "
+            "trajectory": "dt 5e-6, 10 leapfrog steps (11 force evaluations), mass 1, T 1, "
                           "neighbour list rebuilt every step"}
 
 
