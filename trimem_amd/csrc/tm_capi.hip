@@ -544,7 +544,8 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         HIPCHECK(hipMemsetAsync(L.pc_disp.p, 0x7F, sizeof(double) * kDispBlocks, g_rt.stream));    // ~1e306
         L.pc_ok = false;
     }
-    sync();
+    // no wait here: everything that reads the list is queued behind it on the
+    // library stream (the host already waited once, for the bounding box)
 }
 
 void check_params(const tm_params& p)
