@@ -1176,7 +1176,6 @@ template <class T, int kThreads>
 __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
-    const NListView& nl = A.nl;
     const EParams& p = A.p;
     const int t = lb * kThreads + threadIdx.x;
     const int v = t / kGroup, lane = t % kGroup;
