@@ -512,6 +512,7 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         check_launch();
         v.vcand = L.vcand.p;
         v.cand = L.cand.p;
+        v.cand_start = L.cand_start.p;
     }
     if (L.kind == 1) {
         L.adj_counts.alloc(n + 1);

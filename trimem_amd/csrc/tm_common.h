@@ -102,6 +102,7 @@ struct NListView {
     // per cell; vcand[v] is the list of the build-time cell of v
     const int2* __restrict__ vcand;        // [N] range in cand of the build-time cell of each vertex
     const int* __restrict__ cand;
+    const int* __restrict__ cand_start;    // [N + 1] per slot of cell_verts; a cell's list starts at its first slot
     // per vertex, bit i set if the i-th candidate of its list is excluded
     // (topology only; 2 x 64 bits, the first 128 candidates); null if not
     // built. Read only by the vertex-group query of the energy passes, which

@@ -407,8 +407,17 @@ __device__ __forceinline__ void group_neighbours(const MeshView& m, const NListV
             }
             return;
         }
-        // moved vertex: the 9 rows of x-consecutive cells of the stencil of
-        // its current cell, each strided over the lanes
+        // moved vertex: the stored list of its current cell when that cell was
+        // occupied at the build (the same candidates as its 9 stencil rows,
+        // in the same order), else the 9 rows of x-consecutive cells of its
+        // stencil, each strided over the lanes
+        if (inside && nl.cand_start) {
+            const int s0 = nl.cell_start[flat];
+            if (nl.cell_start[flat + 1] > s0) {
+                scan(nl.cand, nl.cand_start[s0], nl.cand_start[s0 + 1]);
+                return;
+            }
+        }
         const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
         if (i0 > i1) return;
         for (int r = 0; r < 9; r++) {
