@@ -265,7 +265,7 @@ def replica_rates(precision, evals, hmc_steps, hip, stream):
             "hmc_group_steps_per_s_graph": 1e3 / ms_graph, "hmc_group_steps_per_s_direct": 1e3 / ms_nograph,
             "hmc_acceptance": float(st["accepted"].sum() / max(st["steps"].sum(), 1)),
             "swap_acceptance": float(st["swaps_accepted"].sum() / max(st["swaps_tried"].sum(), 1)),
-            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "rep_fused")},
+            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "rep_fused", "pair_cache")},
             "workload": "64 icosphere meshes, levels 3-6 from seed, vfrac U[0.6,0.95], kappa_b log-uniform "
                         "[10,100] ladder; HMC dt 1e-5, 10 leapfrog steps, exchange every 10 steps"}
 

@@ -226,8 +226,9 @@ int tm_hmc_result(tm_hmc* hmc, tm_hmc_outcome* out);
  * small meshes, whose launches do not fill the GPU.  The autotuner times
  * every shape on the caller's mesh (HIP events, the mean of 5 launches after
  * 2 untimed ones), then the fused form against the best separate one, and
- * keeps the fastest.  Replica groups always launch the query apart and in
- * full (their rep_fused and pair_cache read 0). */
+ * keeps the fastest.  Replica groups launch the query apart (their
+ * rep_fused reads 0); their pair cache keeps a recording flag and a
+ * displacement check per replica on the device, so graph replays stay valid. */
 typedef struct tm_tuning {
     int rep_threads;     /* repulsion query: 64, 128, 256 or 512 threads per block */
     int rep_batch;       /* candidates in flight per lane: 1, 2, 4 or 8 */

@@ -224,3 +224,48 @@ def test_grouped_shapes_and_autotune(core):
     print("group autotuned", best, flush=True)
     R.eval()
     assert np.array_equal(R.globals(), ref[0])
+
+
+def test_grouped_pair_cache_is_bit_identical(core):
+    """Replica groups replay recorded candidate lists per replica (device
+    recording flags, per-replica displacement checks); a group with the
+    cache and one without agree bit for bit through HMC trajectories, a
+    flip on one replica's mesh (its exclusions change) and a list rebuild.
+    The Verlet-list replica takes the full query in both."""
+    def group(cache):
+        reps = mixed_set(core, "mixed")
+        R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+        t = R.tuning
+        t["pair_cache"] = cache
+        R.tuning = t
+        R.hmc_setup(mass=1.0, time_step=2e-3, temperature=1.0, seeds=list(range(1, len(reps) + 1)),
+                    num_integration_steps=6)
+        return R, reps
+
+    on, reps_on = group(1)
+    off, reps_off = group(0)
+    assert on.tuning["pair_cache"] == 1 and off.tuning["pair_cache"] == 0
+
+    def same(what):
+        on.eval()
+        off.eval()
+        assert np.array_equal(on.globals(), off.globals()), what
+        for i in range(len(reps_on)):
+            assert np.array_equal(on.gradient(i), off.gradient(i)), (what, i)
+            assert np.array_equal(reps_on[i][0].points(), reps_off[i][0].points()), (what, i)
+
+    same("start")
+    on.hmc_run(4)
+    off.hmc_run(4)
+    same("after hmc")
+    for (m, e) in (reps_on[1], reps_off[1]):
+        assert core.flip(m, e, 0.3, seed=17) > 0
+    same("after flip")
+    on.hmc_run(3)
+    off.hmc_run(3)
+    same("after flip and hmc")
+    on.update_repulsion()
+    off.update_repulsion()
+    on.hmc_run(3)
+    off.hmc_run(3)
+    same("after rebuild and hmc")

@@ -646,6 +646,25 @@ void rebuild_nlist(tm_energy& E, tm_mesh& M)
     std::swap(E.nlist, E.spare);
 }
 
+// The pair-cache buffers of the energy manager's cell list (x_snap null when
+// it has none: a Verlet list, or no repulsion); the caller adds the
+// displacement partials and the recording flag.
+PairCache pair_cache_of(tm_energy& E, tm_mesh& M)
+{
+    PairCache pc{};
+    tm_nlist& L = *E.nlist;
+    if (L.kind == 0 && L.pc_x.p && L.nv == M.nv && E.params.rep_lc1 > 0) {
+        const double skin = kPairSkin * E.params.rep_lc1;
+        pc.x_snap = L.pc_x.p;
+        pc.cell = L.pc_cell.p;
+        pc.cnt = L.pc_cnt.p;
+        pc.ent = L.pc_ent.p;
+        pc.skin = skin;
+        pc.limit2 = (kPairLimit * skin) * (kPairLimit * skin);
+    }
+    return pc;
+}
+
 PassBuffers buffers(tm_energy& E, tm_mesh& M)
 {
     PassBuffers b{};
@@ -660,16 +679,13 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.list = M.generic.p;
     b.nlist = M.n_generic;
     tm_nlist& L = *E.nlist;
-    if (E.tune.pair_cache && L.kind == 0 && L.pc_x.p && L.nv == M.nv && E.params.rep_lc1 > 0) {
-        const double skin = kPairSkin * E.params.rep_lc1;
-        b.pc.x_snap = L.pc_x.p;
-        b.pc.cell = L.pc_cell.p;
-        b.pc.cnt = L.pc_cnt.p;
-        b.pc.ent = L.pc_ent.p;
-        b.pc.disp_part = L.pc_disp.p;
-        b.pc.skin = skin;
-        b.pc.limit2 = (kPairLimit * skin) * (kPairLimit * skin);
-        b.pc.force = L.pc_ok ? 0 : 1;
+    if (E.tune.pair_cache) {
+        b.pc = pair_cache_of(E, M);
+        if (b.pc.x_snap) {
+            b.pc.disp_part = L.pc_disp.p;
+            b.pc.disp_n = kDispBlocks;
+            b.pc.force = L.pc_ok ? 0 : 1;
+        }
     }
     return b;
 }
@@ -1376,11 +1392,22 @@ struct tm_replicas {
     Tuning tune;
     DevBuf<int> d_block0_rep, d_block0_p2;
     int total_rep = 0, total_p2 = 0;
+    // pair cache: per-replica recording flags (device; the reduction clears
+    // them), displacement partials, and the prefix tables of the
+    // displacement check and of the cached query
+    DevBuf<int> force_flags, d_block0_disp, d_block0_rc;
+    DevBuf<double> disp_parts;
+    std::vector<int> h_ones;
+    int total_disp = 0, total_rc = 0;
     GroupLaunch group() const
     {
-        return GroupLaunch{d_args.p, h_args.data(), R, d_block0.p, total_blocks, d_block0_rep.p, total_rep,
-                           d_block0_p2.p, total_p2, tune, mixed};
+        return GroupLaunch{d_args.p,  h_args.data(), R, d_block0.p, total_blocks, d_block0_rep.p, total_rep,
+                           d_block0_p2.p, total_p2, tune, mixed, d_block0_disp.p, total_disp, d_block0_rc.p,
+                           tune.pair_cache ? total_rc : 0};
     }
+    // every replica records at its next evaluation
+    void force_all() { h2d(force_flags.p, h_ones.data(), sizeof(int) * R); }
+    void force(int r) { h2d(force_flags.p + r, h_ones.data() + r, sizeof(int)); }
     // HMC
     bool hmc_ready = false;
     int L = 0;
@@ -1429,6 +1456,20 @@ void refresh_replica_args(tm_replicas& S)
     }
     S.h_args.resize(S.R);
     S.h_block0.assign(S.R + 1, 0);
+    // displacement partials: one per 4096 vertices (at most 64) per replica;
+    // cached query: pass1_blocks / 8 blocks of 256 vertices per replica
+    std::vector<int> b0d(S.R + 1, 0), b0c(S.R + 1, 0);
+    for (int r = 0; r < S.R; r++) {
+        b0d[r + 1] = b0d[r] + std::min(std::max((S.meshes[r]->nv + 4095) / 4096, 1), 64);
+        b0c[r + 1] = b0c[r] + pass1_blocks(S.meshes[r]->nv) / 8;
+    }
+    S.total_disp = b0d[S.R];
+    S.total_rc = (b0c[S.R] + 7) / 8 * 8;    // a multiple of the 8 XCDs; padding blocks fall past the last replica's vertices
+    S.d_block0_disp.alloc(S.R + 1);
+    S.d_block0_rc.alloc(S.R + 1);
+    S.disp_parts.alloc(S.total_disp);
+    S.force_flags.alloc(S.R);
+    S.h_ones.assign(S.R, 1);
     for (int r = 0; r < S.R; r++) {
         tm_mesh& M = *S.meshes[r];
         tm_energy& E = *S.ems[r];
@@ -1438,7 +1479,12 @@ void refresh_replica_args(tm_replicas& S)
         EParams ep = eparams_of(E);
         if (S.has_ladder) ep.kappa_b = S.h_ladder[slot_of[r]];
         PassBuffers b = buffers(E, M);
-        b.pc = PairCache{};    // the cache serves single-mesh launches only
+        b.pc = pair_cache_of(E, M);
+        if (b.pc.x_snap) {
+            b.pc.disp_part = S.disp_parts.p + b0d[r];
+            b.pc.disp_n = b0d[r + 1] - b0d[r];
+            b.pc.dforce = S.force_flags.p + r;
+        }
         S.h_args[r] = pass_args(M.view(), E.nlist->view(), ep, b, nullptr, E.grad.p);
         S.h_block0[r + 1] = S.h_block0[r] + S.h_args[r].nblocks;
     }
@@ -1447,7 +1493,9 @@ void refresh_replica_args(tm_replicas& S)
     S.d_block0.alloc(S.R + 1);
     h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
     h2d(S.d_block0.p, S.h_block0.data(), sizeof(int) * (S.R + 1));
-    set_group_shape(S, S.tune);
+    h2d(S.d_block0_disp.p, b0d.data(), sizeof(int) * (S.R + 1));
+    h2d(S.d_block0_rc.p, b0c.data(), sizeof(int) * (S.R + 1));
+    set_group_shape(S, S.tune);    // raises every recording flag
 }
 
 // block prefix tables of the repulsion query and pass 2 for tuning t
@@ -1460,7 +1508,7 @@ void set_group_shape(tm_replicas& S, const Tuning& t)
     }
     S.tune = t;
     S.tune.rep_fused = 0;     // grouped launches keep the query apart
-    S.tune.pair_cache = 0;    // and run it in full
+    S.force_all();            // (a cache switched back on must not replay lists it did not keep)
     S.total_rep = rep[S.R];
     S.total_p2 = p2[S.R];
     S.d_block0_rep.alloc(S.R + 1);
@@ -1481,7 +1529,12 @@ void replicas_device_points(tm_replicas& S)
 // the mask buffers, whose addresses do not change)
 void replicas_masks(tm_replicas& S)
 {
-    for (int r = 0; r < S.R; r++) ensure_masks(*S.ems[r]->nlist, *S.meshes[r]);
+    for (int r = 0; r < S.R; r++) {
+        tm_nlist& L = *S.ems[r]->nlist;
+        const unsigned before = L.mask_topo;
+        ensure_masks(L, *S.meshes[r]);
+        if (L.mask_topo != before) S.force(r);    // new topology: exclusion may have changed
+    }
 }
 
 void replicas_hmc_step(tm_replicas& S)

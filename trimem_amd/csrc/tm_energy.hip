@@ -483,11 +483,11 @@ __device__ __forceinline__ int query_cell(const NListView& nl, d3 pi)
 // more since the recording), 1 to replay.  Contains a barrier.
 __device__ __forceinline__ int pair_cache_mode(const PairCache& pc)
 {
-    if (pc.force) return 2;
+    if (pc.force || (pc.dforce && *pc.dforce)) return 2;
     __shared__ int s_mode;
     if (threadIdx.x < 64) {
         double mx = 0.0;
-        for (int i = threadIdx.x; i < kDispBlocks; i += 64) mx = fmax(mx, pc.disp_part[i]);
+        for (int i = threadIdx.x; i < pc.disp_n; i += 64) mx = fmax(mx, pc.disp_part[i]);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
         if (threadIdx.x == 0) s_mode = mx < pc.limit2 ? 1 : 2;
@@ -553,12 +553,12 @@ __device__ __forceinline__ bool cached_neighbours(const MeshView& m, const NList
 
 // max |x - x_snap|^2 over the vertices, one partial per block (NaN counts as
 // infinite, which forces a recording)
-__global__ void __launch_bounds__(256) k_disp(const double* __restrict__ x, const double* __restrict__ xs, int n,
-                                              double* __restrict__ part)
+__device__ __forceinline__ void disp_body(const double* __restrict__ x, const double* __restrict__ xs, int n,
+                                          double* __restrict__ part, int nblk, int blk)
 {
     __shared__ double s_red[4];
     double mx = 0.0;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    for (int v = blk * blockDim.x + threadIdx.x; v < n; v += nblk * blockDim.x) {
         const double dx = x[3 * v] - xs[3 * v];
         const double dy = x[3 * v + 1] - xs[3 * v + 1];
         const double dz = x[3 * v + 2] - xs[3 * v + 2];
@@ -570,7 +570,13 @@ __global__ void __launch_bounds__(256) k_disp(const double* __restrict__ x, cons
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mx;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+    if (threadIdx.x == 0) part[blk] = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+}
+
+__global__ void __launch_bounds__(256) k_disp(const double* __restrict__ x, const double* __restrict__ xs, int n,
+                                              double* __restrict__ part)
+{
+    disp_body(x, xs, n, part, gridDim.x, blockIdx.x);
 }
 
 // Exclusion masks of the stored stencil lists: bit i of vertex v is set if
@@ -772,6 +778,18 @@ __global__ void __launch_bounds__(256) k_pass1_apart(PassArgs A)
     pass1_body<T, true>(A, logical_block());
 }
 
+// replica groups with the cached query: replicas with repulsion take their
+// repulsion column from it
+template <class T>
+__global__ void __launch_bounds__(256) k_pass1_apart_grouped(const PassArgs* __restrict__ A,
+                                                             const int* __restrict__ block0, int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    if (A[r].rep_v) pass1_body<T, true>(A[r], lb - block0[r]);
+    else pass1_body<T, false>(A[r], lb - block0[r]);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
                                                        int R)
@@ -856,20 +874,23 @@ __global__ void __launch_bounds__(kThreads) k_rep(PassArgs A)
 // mode says so) over that list.  Replays are the common case and want few,
 // full blocks; a recording launch the device decides on (some vertex moved
 // too far) is slower than k_rep, as each group takes 8 vertices in turn.
-constexpr int kVertsPerBlock = 256;
+// Without cache buffers (a replica with a Verlet list in a cached group
+// launch) every ring vertex takes phase 2 without recording (mode 0).
+constexpr int kVertsPerBlock = kCachedVerts;
 template <int kBatch>
-__global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
+__device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
 {
+    if (!A.rep_v) return;
     const MeshView& m = A.m;
     const EParams& p = A.p;
     const PairCache& pc = A.pc;
-    const int mode = pair_cache_mode(pc);    // block-uniform, 1 replay, 2 record
+    const int mode = pc.x_snap ? pair_cache_mode(pc) : 0;    // block-uniform: 1 replay, 2 record, 0 full
     __shared__ int s_list[kVertsPerBlock];
     __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (pr[5] of pass 1)
     __shared__ int s_n;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const int base = logical_block() * kVertsPerBlock;
+    const int base = lb * kVertsPerBlock;
     const int v = base + threadIdx.x;
     if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
     if (threadIdx.x < kVertsPerBlock && v < m.nv) {
@@ -884,7 +905,7 @@ __global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
             pc.x_snap[3 * v + 1] = xv.y;
             pc.x_snap[3 * v + 2] = xv.z;
             if (!ring) pc.cell[v] = -1;
-        } else if (ring) {
+        } else if (ring && mode == 1) {
             const int cell = pc.cell[v];
             const unsigned word = pc.cnt[v];
             const bool replay = cell >= 0 && query_cell(A.nl, xv) == cell;
@@ -967,6 +988,32 @@ __global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
         const int lb1 = base / 32 + c;
         if (lb1 < A.nblocks) A.partials[5 * A.nblocks + lb1] = a;
     }
+}
+
+template <int kBatch>
+__global__ void __launch_bounds__(256) k_rep_cached(PassArgs A)
+{
+    rep_cached_body<kBatch>(A, logical_block());
+}
+
+// replica groups: block0 gives each replica pass1_blocks / 8 blocks (its
+// pass-1 partial rows, 32 vertices each, in blocks of 256 vertices)
+template <int kBatch>
+__global__ void __launch_bounds__(256) k_rep_cached_grouped(const PassArgs* __restrict__ A,
+                                                            const int* __restrict__ block0, int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    rep_cached_body<kBatch>(A[r], lb - block0[r]);
+}
+
+__global__ void __launch_bounds__(256) k_disp_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
+                                                      int R)
+{
+    const int r = find_replica(block0, R, blockIdx.x);
+    const PassArgs& a = A[r];
+    if (!a.rep_v || !a.pc.x_snap) return;
+    disp_body(a.m.x, a.pc.x_snap, a.m.nv, a.pc.disp_part, block0[r + 1] - block0[r], blockIdx.x - block0[r]);
 }
 
 // A recording launch the host asks for (new list or topology): the layout
@@ -1063,6 +1110,7 @@ __global__ void __launch_bounds__(1024) k_finalize_grouped(const PassArgs* __res
     __shared__ double s_out[6];
     const PassArgs& a = A[blockIdx.x];
     finalize_globals(a.partials, a.nblocks, a.p, a.globals, s_red, s_out);
+    if (threadIdx.x == 0 && a.pc.dforce) *a.pc.dforce = 0;    // the query of this evaluation recorded
 }
 
 // OpenMesh-semantics properties of the listed vertices (boundary and
@@ -1439,7 +1487,19 @@ void launch_rep_grouped(const GroupLaunch& g, hipStream_t s)
 {
     bool rep = false;
     for (int r = 0; r < g.R; r++) rep = rep || g.h_args[r].rep_v;
-    if (rep) rep_dispatch(g.tune, true, g.h_args[0], &g, s);
+    if (!rep) return;
+    if (g.total_rc > 0) {
+        hipLaunchKernelGGL(k_disp_grouped, dim3(g.total_disp), dim3(256), 0, s, g.d_args, g.d_block0_disp, g.R);
+        const dim3 grid(g.total_rc), block(256);
+        switch (g.tune.rep_batch) {
+        case 1: hipLaunchKernelGGL(k_rep_cached_grouped<1>, grid, block, 0, s, g.d_args, g.d_block0_rc, g.R); break;
+        case 2: hipLaunchKernelGGL(k_rep_cached_grouped<2>, grid, block, 0, s, g.d_args, g.d_block0_rc, g.R); break;
+        case 8: hipLaunchKernelGGL(k_rep_cached_grouped<8>, grid, block, 0, s, g.d_args, g.d_block0_rc, g.R); break;
+        default: hipLaunchKernelGGL(k_rep_cached_grouped<4>, grid, block, 0, s, g.d_args, g.d_block0_rc, g.R); break;
+        }
+        return;
+    }
+    rep_dispatch(g.tune, true, g.h_args[0], &g, s);
 }
 
 void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
@@ -1450,8 +1510,14 @@ void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
                                r);
     launch_rep_grouped(g, s);
     dim3 grid(g.total_p1), block(256);
-    if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    if (g.total_rc > 0) {
+        if (g.mixed) hipLaunchKernelGGL(k_pass1_apart_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        else hipLaunchKernelGGL(k_pass1_apart_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    } else if (g.mixed) {
+        hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    } else {
+        hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    }
     hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(1024), 0, s, g.d_args);
 }
 

@@ -27,11 +27,14 @@ struct PairCache {
     int* cell;              // [N] cell id at the recording, -1: replay not allowed
     unsigned* cnt;          // [N] recorded candidates of each lane, 4 bits per lane
     int* ent;               // [8 kPairCap N] ids, lane-major per vertex
-    double* disp_part;      // [kDispBlocks] max |x - x_snap|^2 per block of k_disp
+    double* disp_part;      // [disp_n] max |x - x_snap|^2 per block of k_disp
+    int disp_n;
+    int* dforce;            // replica groups: device flag, nonzero records (the reduction clears it)
     double skin;
     double limit2;          // replay while max |x - x_snap|^2 < limit2
-    int force;              // 1: record in this launch
+    int force;              // single meshes: 1 records in this launch
 };
+constexpr int kCachedVerts = 256;    // vertices per block of the cached query
 
 // energy passes (tm_energy.hip)
 struct PassBuffers {
@@ -108,6 +111,12 @@ struct GroupLaunch {
     int total_p2;
     Tuning tune;
     int mixed;
+    // with the pair cache (h_args[r].pc.x_snap set): the displacement check
+    // and the cached query, one prefix table each
+    const int* d_block0_disp;
+    int total_disp;
+    const int* d_block0_rc;
+    int total_rc;
 };
 void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s);
 void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s);
