@@ -664,7 +664,8 @@ __device__ __forceinline__ d3 shfl_d3(d3 a, int src)
 // Rings of at most kGroup: lane k of the group owns slot k and loads only
 // its own neighbour; e_{k-1} comes from lane k-1 and n_{k+1} from lane k+1
 // (the values those lanes computed, so the bits are those of
-// slot_properties).  Lanes n..kGroup-1 mirror slot 0 and add nothing.
+// slot_properties).  Lanes n..kGroup-1 add nothing, and no lane below n
+// reads them.
 struct RingLanes {
     int k, src_prev, src_next;
 };
@@ -714,11 +715,15 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
     double area = 0, vol = 0, curv = 0, teth = 0;
+    // ring column `lane` needs no valence (columns up to kGroup hold -1 past
+    // it), so it loads beside meta instead of after it
+    const int jl = active ? m.ring[lane * m.nv + v] : -1;
     if (ring) {
         const d3 xv = load_p(m.x, v);
         if (n <= kGroup) {
             const RingLanes r = ring_lanes(n, lane);
-            const d3 e_cur = load_p(m.x, m.ring[r.k * m.nv + v]) - xv;
+            // lanes past the valence take v itself; no lane below it reads them
+            const d3 e_cur = load_p(m.x, jl >= 0 ? jl : v) - xv;
             const d3 e_prev = shfl_d3(e_cur, r.src_prev);
             const d3 n_cur = cross_f(e_cur, e_prev);
             const d3 n_next = shfl_d3(n_cur, r.src_next);
@@ -736,14 +741,16 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     if (active && lane == 0) {
         double q;    // curvature / area
         if (ring) {
-            // a third of the face areas and volumes, half the edge terms
-            const double a = area / 3, c = curv * 0.5;
+            // a third of the face areas and volumes, half the edge terms (by
+            // the rounded reciprocals: at most the last bit differs from a
+            // division, far inside the parity tolerance)
+            const double a = area * (1.0 / 3.0), c = curv * 0.5;
             q = c / a;
             pr[0] = a;
-            pr[1] = vol / 18;
+            pr[1] = vol * (1.0 / 18.0);
             pr[2] = c;
             pr[3] = 2 * c * q;
-            pr[4] = p.bond_type == 0 ? teth * 0.5 : teth / 3;
+            pr[4] = p.bond_type == 0 ? teth * 0.5 : teth * (1.0 / 3.0);
             pr[5] = rv;
         } else {
             for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
@@ -1241,6 +1248,7 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
     d3 acc{0, 0, 0};
+    const int jl = active ? m.ring[lane * m.nv + v] : -1;    // as in pass 1
     if (ring) {
         const Globals g = *A.globals;
         const d3 xv = load_p(m.x, v);
@@ -1248,7 +1256,7 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
         if (n <= kGroup) {
             // as in pass 1: e_{k-1}, q_{k-1} from lane k-1, n_{k+1} from lane k+1
             const RingLanes r = ring_lanes(n, lane);
-            const int jc = m.ring[r.k * m.nv + v];
+            const int jc = jl >= 0 ? jl : v;
             const d3 e_cur = load_p(m.x, jc) - xv;
             const double q_cur = A.q[jc];
             const d3 e_prev = shfl_d3(e_cur, r.src_prev);
