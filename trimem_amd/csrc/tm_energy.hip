@@ -891,19 +891,23 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
     const MeshView& m = A.m;
     const EParams& p = A.p;
     const PairCache& pc = A.pc;
+    const int base = lb * kVertsPerBlock;
+    const int v = base + threadIdx.x;
+    const bool mine = threadIdx.x < kVertsPerBlock && v < m.nv;
+    // this thread's vertex data, in flight while the mode is decided
+    const int meta = mine ? m.meta[v] : 0;
+    const d3 xv = mine ? load_p(m.x, v) : d3{0.0, 0.0, 0.0};
+    const int rec_cell = mine && pc.x_snap ? pc.cell[v] : -1;
+    const unsigned rec_word = mine && pc.x_snap ? pc.cnt[v] : 0u;
     const int mode = pc.x_snap ? pair_cache_mode(pc) : 0;    // block-uniform: 1 replay, 2 record, 0 full
     __shared__ int s_list[kVertsPerBlock];
     __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (pr[5] of pass 1)
     __shared__ int s_n;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const int base = lb * kVertsPerBlock;
-    const int v = base + threadIdx.x;
     if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
-    if (threadIdx.x < kVertsPerBlock && v < m.nv) {
-        const int meta = m.meta[v];
+    if (mine) {
         const bool ring = !is_generic(meta);
-        const d3 xv = load_p(m.x, v);
         bool done = !ring;
         double rep = 0.0;
         int cnt = 0;
@@ -913,8 +917,8 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
             pc.x_snap[3 * v + 2] = xv.z;
             if (!ring) pc.cell[v] = -1;
         } else if (ring && mode == 1) {
-            const int cell = pc.cell[v];
-            const unsigned word = pc.cnt[v];
+            const int cell = rec_cell;
+            const unsigned word = rec_word;
             const bool replay = cell >= 0 && query_cell(A.nl, xv) == cell;
             if (replay && word == 0u) {
                 done = true;    // nothing recorded: no pairs (the sums below would all be +0)
