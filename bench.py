@@ -270,6 +270,76 @@ def replica_rates(precision, evals, hmc_steps, hip, stream):
                         "[10,100] ladder; HMC dt 1e-5, 10 leapfrog steps, exchange every 10 steps"}
 
 
+def flip_rates(v, f, params, precision, ratio=0.1, sweeps=3):
+    """Attempted edge flips per second at L7 (flip_ratio 0.1 of the edges per
+    sweep): the serial sweep (one workgroup walks the proposals in order, as
+    flips.cpp:22-80 does) and the parallel sweep (claim-based conflict-free
+    rounds, flips.cpp:82-166)."""
+    from trimem_amd import core
+    out = {"ratio": ratio}
+    for kind, fn in (("serial", core.flip), ("parallel", core.pflip)):
+        mesh = core.TriMesh(v, f)
+        est = core.EnergyManager(mesh, core_params(core, params))
+        est.precision = precision
+        n = int(mesh.n_edges() * ratio)
+        fn(mesh, est, ratio, seed=1)
+        t0 = time.perf_counter()
+        acc = sum(fn(mesh, est, ratio, seed=2 + k) for k in range(sweeps))
+        dt = time.perf_counter() - t0
+        out[kind] = {"attempted_flips_per_s": sweeps * n / dt, "ms_per_sweep": 1e3 * dt / sweeps,
+                     "accepted_per_sweep": acc / sweeps, "proposals_per_sweep": n}
+    out["edges"] = int(mesh.n_edges())
+    return out
+
+
+def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=20261205):
+    """The combined configuration's meshes on one GPU: 32 icosphere L7 meshes
+    (163842 vertices each, jittered 1% from their own seeds), volume
+    fractions U[0.55, 0.9], kappa_b log-uniform on [20, 80]; the rate of
+    grouped energy+gradient evaluations of all of them and the algorithmic
+    HBM rate (their working set is several times the Infinity Cache)."""
+    from trimem_amd import capi, core, meshes
+    L = capi.lib()
+    v0, f = meshes.icosphere(7)
+    f = np.ascontiguousarray(f.astype(np.int32))
+    lbar = meshes.mean_edge_length(v0, f)
+    rng = np.random.default_rng(seed)
+    vfrac = rng.uniform(0.55, 0.9, size=n_replicas)
+    kb = np.sort(np.exp(rng.uniform(np.log(20.0), np.log(80.0), size=n_replicas)))
+    base = workload(1)[3]    # the headline parameters, scaled to this mesh's edge below
+    ms, es = [], []
+    for r in range(n_replicas):
+        v = np.ascontiguousarray(v0 + np.random.default_rng(seed + 1 + r).normal(0.0, 0.01 * lbar, v0.shape))
+        params = dict(base, kappa_b=float(kb[r]), volume_frac=float(vfrac[r]), bond_lc0=1.25 * lbar,
+                      bond_lc1=0.75 * lbar, rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar)
+        m = core.TriMesh(v, f)
+        e = core.EnergyManager(m, core_params(core, params))
+        e.precision = precision
+        ms.append(m)
+        es.append(e)
+    R = core.Replicas(ms, es)
+    tune = R.autotune()
+    nv = np.array([m.n_vertices() for m in ms], dtype=np.float64)
+    nf = np.array([m.n_faces() for m in ms], dtype=np.float64)
+    eval_bytes = float(np.sum((24.0 * nv + 12.0 * nf + 16.0 * nv) * 2 + 24.0 * nv))
+    R.eval()
+    capi.check(L.tm_sync())
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    for _ in range(evals):
+        R.eval()
+    hip.record(e1, stream)
+    ms_group = hip.elapsed_ms(e0, e1) / evals
+    return {"replicas": n_replicas, "vertices": int(nv.sum()), "group_evals_per_s": 1e3 / ms_group,
+            "ms_per_group_eval": ms_group, "replica_evals_per_s": n_replicas * 1e3 / ms_group,
+            "algorithmic_bytes_per_group_eval": eval_bytes, "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9,
+            "hbm_frac": eval_bytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "pair_cache")},
+            "workload": "32 icosphere L7 meshes jittered 1% (own seeds), vfrac U[0.55,0.9], kappa_b "
+                        "log-uniform [20,80], cell-list repulsion excl 2 rc=1.5l; one grouped energy+gradient "
+                        "evaluation of all"}
+
+
 def ess(x):
     """Effective sample size of a scalar chain (Geyer's initial monotone
     sequence estimator of the integrated autocorrelation time)."""
@@ -472,6 +542,8 @@ def main():
                     help="1: include the surrogate-vs-plain HMC block (L6), 0: skip")
     ap.add_argument("--replica-evals", type=int, default=40,
                     help="grouped evaluations of the 64-replica set timed for the replicas block (0: skip)")
+    ap.add_argument("--flips", type=int, default=1, help="attempted flips/s of serial and parallel sweeps")
+    ap.add_argument("--combined", type=int, default=1, help="grouped evaluations of 32 L7 meshes")
     ap.add_argument("--hmc-steps", type=int, default=40,
                     help="device-resident HMC steps timed for the hmc block (0: skip)")
     ap.add_argument("--summarize", nargs="+", metavar="DIR",
@@ -628,6 +700,10 @@ def main():
         out["surrogate"] = surrogate_rates(args.precision)
     if args.replica_evals > 0:
         out["replicas"] = replica_rates(args.precision, args.replica_evals, args.hmc_steps, hip, stream)
+    if args.flips:
+        out["flips"] = flip_rates(v, f, params, args.precision)
+    if args.combined:
+        out["combined"] = combined_rates(args.precision, hip, stream)
     if not args.no_cpu_baseline:
         cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",
