@@ -259,18 +259,22 @@ def test_mesh_utils(core):
 
 
 # ---------------------------------------------------------------- flips ----
-def flip_case(level, seed, rep):
+def flip_case(level, seed, rep, nlist=0, excl=1):
     v, f, lbar = M.jittered_icosphere(level, seed=seed, rel_sigma=0.05)
     kw = dict(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0, kappa_r=1.0,
               area_frac=1.0, volume_frac=1.0, curvature_frac=1.0, bond_lc0=1.15 * lbar, bond_lc1=0.85 * lbar)
     if rep:
-        kw.update(rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar, rep_exclusion=1)
+        kw.update(rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar, rep_exclusion=excl, rep_nlist=nlist)
     return v, f, O.Params(**kw)
 
 
-@pytest.mark.parametrize("level,rep", [(2, False), (3, False), (3, True)])
-def test_flip_serial_injected_parity(core, level, rep):
-    v, f, p = flip_case(level, 11 + level, rep)
+@pytest.mark.parametrize("level,rep,nlist,excl", [(2, False, 0, 1), (3, False, 0, 1), (3, True, 0, 1),
+                                                  (3, True, 0, 2), (3, True, 0, 0), (3, True, 1, 0), (3, True, 1, 1)])
+def test_flip_serial_injected_parity(core, level, rep, nlist, excl):
+    """Accepted count, uniforms used and the face list after a serial sweep
+    over an injected proposal/uniform stream equal the oracle's, for the
+    cell list at exclusion levels 0-2 and the Verlet list."""
+    v, f, p = flip_case(level, 11 + level + 3 * (excl - 1) + 7 * nlist, rep, nlist, excl)
     om, oe, gm, ge = make_pair(core, v, f, p)
     rng = np.random.default_rng(level)
     n = int(gm.n_edges() * 0.5)

@@ -4,9 +4,9 @@
 // proposal depends on the topology left by the previous one, so it runs as
 // one workgroup walking the proposal stream in order.  For each proposal:
 //   * lane 0 tests OpenMesh is_flip_ok and the boundary condition;
-//   * lanes 0-3 evaluate vertex_properties of the four patch vertices
-//     (flip_utils.cpp:19-41) in parallel with the OpenMesh-semantics walker,
-//     including their repulsion queries;
+//   * the workgroup evaluates vertex_properties of the four patch vertices
+//     (flip_utils.cpp:19-41), 16 lanes per vertex (coop_patch), including
+//     their repulsion queries;
 //   * lane 0 flips (OpenMesh TriConnectivity::flip rewiring), the lanes
 //     re-evaluate the patch, and lane 0 applies the FP64 Metropolis test
 //     u <= (dE < 0 ? 1 : exp(-dE)) and flips back on rejection.
@@ -135,25 +135,256 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
     return e;
 }
 
-// Patch properties: lanes 0-3 each evaluate one of the four vertices
-// to(2e), to(next(2e)), to(2e+1), to(next(2e+1)); the sum is accumulated in
-// that order in s_sum by thread 0.
-__device__ __forceinline__ void patch_properties(const FlipMeshView& fm, const NListView& nl, const EParams& p,
-                                                 int e, double (*s_vp)[6], double* s_sum)
+// ------------------------------------------ cooperative patch evaluation --
+// vertex_properties (mesh_properties.cpp:12-49) of the four vertices of an
+// edge's flip patch (flip_utils.cpp:19-41), to(2e), to(next(2e)), to(2e+1),
+// to(next(2e+1)), by one 64-lane workgroup: lanes 16g .. 16g+15 evaluate
+// patch vertex g.
+//   phase 1  lane 0 of the group walks the one-ring (clockwise from v_he,
+//            as the circulator does); every lane locates the candidate runs
+//            of the repulsion query;
+//   phase 2  the lanes compute the per-halfedge terms and, for exclusion
+//            level 2, gather the one-rings of the ring vertices into LDS;
+//   phase 3  the lanes test the candidates in chunks of 64, the exclusion
+//            test reading LDS instead of walking halfedges per candidate;
+//   phase 4  lane 0 adds the terms of each accumulator in the order the
+//            single-thread walker (om_vertex_properties) adds them, so every
+//            value is bit-identical to it; thread 0 adds the four vertices.
+// A vertex whose valence exceeds kMaxValence is evaluated by its lane 0
+// with om_vertex_properties; a ring vertex whose valence exceeds it sends
+// that group's exclusion tests to om_excluded.
+constexpr int kCoopLanes = 16;
+constexpr int kCoopChunk = 4 * kCoopLanes;
+
+struct CoopShared {
+    int h[4][kMaxValence];              // outgoing halfedges, circulator order
+    int nb[4][kMaxValence];             // their target vertices (the one-ring)
+    int bnd[4][kMaxValence];            // halfedge on the boundary
+    int n[4];                           // valence; -1 above kMaxValence
+    int total[4];                       // repulsion candidates
+    int slow[4];                        // exclusion tests by om_excluded
+    double slot[4][kMaxValence][4];     // area, volume, edge curvature, tether
+    int ring2[4][kMaxValence][kMaxValence];
+    int ring2_n[4][kMaxValence];
+    double rep[4][kCoopChunk];
+    double vp[4][6];
+    double sum[6];
+};
+
+__device__ __forceinline__ int patch_vertex(const Topo& t, int e, int g)
 {
-    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
-    int lane = threadIdx.x;
-    if (lane < 4) {
-        int h = 2 * e + (lane >> 1);
-        int ve = (lane & 1) ? t.to(t.next(h)) : t.to(h);
-        VP r = om_vertex_properties(t, nl, p, ve);
-        for (int k = 0; k < 6; k++) s_vp[lane][k] = r.v[k];
+    int h = 2 * e + (g >> 1);
+    return (g & 1) ? t.to(t.next(h)) : t.to(h);
+}
+
+// The candidate runs of a vertex's repulsion query, in the order
+// nlist_visit walks them: the nine (dj, dk) rows of three consecutive cells
+// of cell_verts, or the vertex's adj range of the Verlet list.  Rows that
+// fall outside the grid are empty runs, so every index is a constant after
+// unrolling and the runs stay in registers.
+struct CandRuns {
+    int s[9], len[9];
+    int total;
+};
+
+__device__ __forceinline__ CandRuns cand_runs(const NListView& nl, int i, d3 pi)
+{
+    CandRuns r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.s[k] = r.len[k] = 0;
+    if (nl.kind == 0) {
+        int c[3];
+        cell_coords(nl, pi, c);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int kk = c[2] + k / 3 - 1, jj = c[1] + k % 3 - 1;
+            int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
+            if (kk >= 0 && kk < nl.shape[2] && jj >= 0 && jj < nl.shape[1] && i0 <= i1) {
+                int base = jj * nl.strides[1] + kk * nl.strides[2];
+                r.s[k] = nl.cell_start[base + i0];
+                r.len[k] = nl.cell_start[base + i1 + 1] - r.s[k];
+            }
+        }
+    } else if (nl.kind == 1) {
+        r.s[0] = nl.adj_start[i];
+        r.len[0] = nl.adj_start[i + 1] - r.s[0];
+    }
+    r.total = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.total += r.len[k];
+    return r;
+}
+
+// Slot of the c-th candidate (0 <= c < total) in cell_verts / adj.
+__device__ __forceinline__ int cand_slot(const CandRuns& r, int c)
+{
+    int q = -1;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        if (q < 0 && c < r.len[k]) q = r.s[k] + c;
+        if (q < 0) c -= r.len[k];
+    }
+    return q;
+}
+
+__device__ __forceinline__ bool coop_excluded(const CoopShared& S, int g, int n, int excl, int j)
+{
+    if (excl <= 0) return false;
+    for (int k = 0; k < n; k++) {
+        if (S.nb[g][k] == j) return true;
+        if (excl > 1) {
+            const int m = S.ring2_n[g][k];
+            for (int q = 0; q < m; q++)
+                if (S.ring2[g][k][q] == j) return true;
+        }
+    }
+    return false;
+}
+
+// Called by all 64 threads of the workgroup; S.vp and S.sum hold the result
+// on return.
+__device__ void coop_patch(const Topo& t, const NListView& nl, const EParams& p, int e, CoopShared& S)
+{
+    const int g = threadIdx.x / kCoopLanes, l = threadIdx.x % kCoopLanes;
+    const int ve = patch_vertex(t, e, g);
+    const bool rep_on = p.rep_lc1 > 0 && nl.kind >= 0;
+    const int excl = nl.kind == 0 ? nl.exclusion : 0;
+    const d3 pi = t.P(ve);
+
+    // phase 1
+    CandRuns runs = cand_runs(nl, ve, pi);
+    if (l == 0) {
+        int n = 0;
+        int h0 = t.v_he[ve];
+        if (h0 >= 0) {
+            int h = h0;
+            do {
+                if (n == kMaxValence) {
+                    n = -1;
+                    break;
+                }
+                S.h[g][n] = h;
+                S.nb[g][n] = t.to(h);
+                n++;
+                h = t.cw(h);
+            } while (h != h0);
+        }
+        S.n[g] = n;
+        S.total[g] = (rep_on && n >= 0) ? runs.total : 0;
+        S.slow[g] = 0;
     }
     __syncthreads();
-    if (lane == 0) {
-        for (int k = 0; k < 6; k++) s_sum[k] = 0.0;
+    const int n = S.n[g];
+
+    // phase 2
+    for (int s = l; s < n; s += kCoopLanes) {
+        const int h = S.h[g][s];
+        const int b = t.he_boundary(h);
+        S.bnd[g][s] = b;
+        if (!b) {
+            double el = mu_edge_length(t, h);
+            double da = om_dihedral_angle(t, h);
+            S.slot[g][s][0] = mu_face_area(t, h);
+            S.slot[g][s][1] = mu_face_volume(t, h);
+            S.slot[g][s][2] = 0.5 * el * da;
+            S.slot[g][s][3] = p.bond_type == 0 ? bond_edge_value(el, p) : bond_area_value(mu_face_area(t, h), p);
+        }
+        if (rep_on && excl > 1) {
+            const int g0 = t.v_he[S.nb[g][s]];
+            int m = 0;
+            if (g0 >= 0) {
+                int gh = g0;
+                do {
+                    if (m == kMaxValence) {
+                        S.slow[g] = 1;
+                        break;
+                    }
+                    S.ring2[g][s][m++] = t.to(gh);
+                    gh = t.cw(gh);
+                } while (gh != g0);
+            }
+            S.ring2_n[g][s] = m;
+        }
+    }
+    __syncthreads();
+
+    // phase 3
+    const int total = S.total[g];
+    const int tmax = max(max(S.total[0], S.total[1]), max(S.total[2], S.total[3]));
+    const bool slow = S.slow[g] != 0;
+    const double dmax = p.rep_lc1;
+    const double dmax2 = dmax * dmax * (1.0 + 1e-15);    // prefilter, as in nlist_visit
+    double rep = 0.0;
+    for (int base = 0; base < tmax; base += kCoopChunk) {
+        double val[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int c = base + k * kCoopLanes + l;
+            val[k] = 0.0;
+            if (c < total) {
+                const int q = cand_slot(runs, c);
+                const int j = nl.kind == 0 ? nl.cell_verts[q] : nl.adj[q];
+                const d3 d = pi - t.P(j);
+                const double sq = sqnorm(d);
+                bool in = sq <= dmax2 && sqrt(sq) < dmax;
+                if (in && nl.kind == 0)
+                    in = j != ve && !(slow ? om_excluded(t, ve, j, excl) : coop_excluded(S, g, n, excl, j));
+                if (in) val[k] = rep_value(norm(d), p.rep_lc1, p.rep_r);
+            }
+            S.rep[g][k * kCoopLanes + l] = val[k];
+        }
+        // a zero term leaves the (non-negative) sum unchanged: lane 0 adds
+        // only the lanes whose term is non-zero, in candidate order
+        unsigned long long mask[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) mask[k] = __ballot(val[k] != 0.0);
+        __syncthreads();
+        if (l == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                unsigned bits = (unsigned)(mask[k] >> (kCoopLanes * g)) & 0xffffu;
+                while (bits) {
+                    const int b = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    rep += S.rep[g][k * kCoopLanes + b];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // phase 4
+    if (l == 0) {
+        VP r;
+        if (n < 0) {
+            r = om_vertex_properties(t, nl, p, ve);
+        } else {
+            double area = 0, vol = 0, curv = 0, teth = 0;
+            for (int s = 0; s < n; s++) {
+                if (S.bnd[g][s]) continue;
+                area += S.slot[g][s][0];
+                vol += S.slot[g][s][1];
+                curv += S.slot[g][s][2];
+                teth += S.slot[g][s][3];
+            }
+            area /= 3;
+            vol /= 3;
+            curv /= 2;
+            teth /= (p.bond_type == 0 ? 2 : 3);
+            r.v[0] = area;
+            r.v[1] = vol;
+            r.v[2] = curv;
+            r.v[3] = 2 * curv * curv / area;
+            r.v[4] = teth;
+            r.v[5] = rep;
+        }
+        for (int k = 0; k < 6; k++) S.vp[g][k] = r.v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) S.sum[k] = 0.0;
         for (int i = 0; i < 4; i++)
-            for (int k = 0; k < 6; k++) s_sum[k] += s_vp[i][k];
+            for (int k = 0; k < 6; k++) S.sum[k] += S.vp[i][k];
     }
     __syncthreads();
 }
@@ -164,9 +395,7 @@ __global__ void __launch_bounds__(64) k_flip_serial(FlipMeshView fm, NListView n
                                                     uint64_t seed, int injected, const Globals* __restrict__ g0,
                                                     int* __restrict__ result)
 {
-    __shared__ double s_vp[4][6];
-    __shared__ double s_old[6];
-    __shared__ double s_new[6];
+    __shared__ CoopShared S;
     __shared__ int s_go;
     __shared__ int s_stop;
     const int ne = fm.nh / 2;
@@ -193,16 +422,20 @@ __global__ void __launch_bounds__(64) k_flip_serial(FlipMeshView fm, NListView n
         if (s_stop) break;
         const int e = s_go;
         if (e < 0) continue;
-        patch_properties(fm, nl, p, e, s_vp, s_old);
+        coop_patch(t, nl, p, e, S);
+        double old_sum[6];
         if (threadIdx.x == 0) {
-            for (int k = 0; k < 6; k++) props[k] -= s_old[k];
+            for (int k = 0; k < 6; k++) {
+                old_sum[k] = S.sum[k];
+                props[k] -= old_sum[k];
+            }
             do_flip(fm, e);
         }
         __threadfence_block();
         __syncthreads();
-        patch_properties(fm, nl, p, e, s_vp, s_new);
+        coop_patch(t, nl, p, e, S);
         if (threadIdx.x == 0) {
-            for (int k = 0; k < 6; k++) props[k] += s_new[k];
+            for (int k = 0; k < 6; k++) props[k] += S.sum[k];
             double en = energy_of(props, p);
             double de = en - e0;
             double alpha = de < 0.0 ? 1.0 : exp(-de);
@@ -225,8 +458,8 @@ __global__ void __launch_bounds__(64) k_flip_serial(FlipMeshView fm, NListView n
             } else {
                 do_flip(fm, e);
                 for (int k = 0; k < 6; k++) {
-                    props[k] -= s_new[k];
-                    props[k] += s_old[k];
+                    props[k] -= S.sum[k];
+                    props[k] += old_sum[k];
                 }
             }
         }
@@ -272,25 +505,20 @@ void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EPara
 
 // Edges of the flip patch of edge e (flip_utils.cpp:43-68): for the two
 // endpoints' and the two opposite vertices' outgoing halfedges h, the edges
-// of h and of next(h).
+// of h and of next(h).  patch_vertex(t, e, w) is the w-th of those four
+// vertices; this visits its part of the patch.
 template <class F>
-__device__ __forceinline__ void for_patch_edges(const Topo& t, int e, F&& f)
+__device__ __forceinline__ void for_patch_edges_of(const Topo& t, int e, int w, F&& f)
 {
-    for (int i = 0; i < 2; i++) {
-        int heh = 2 * e + i;
-        for (int w = 0; w < 2; w++) {
-            int vh = t.to(heh);
-            int h0 = t.v_he[vh];
-            int h = h0;
-            int guard = 0;
-            do {
-                f(h >> 1);
-                f(t.next(h) >> 1);
-                h = t.cw(h);
-            } while (h != h0 && ++guard < 4 * kMaxValence);
-            heh = t.next(heh);
-        }
-    }
+    int vh = patch_vertex(t, e, w);
+    int h0 = t.v_he[vh];
+    int h = h0;
+    int guard = 0;
+    do {
+        f(h >> 1);
+        f(t.next(h) >> 1);
+        h = t.cw(h);
+    } while (h != h0 && ++guard < 4 * kMaxValence);
 }
 
 __device__ __forceinline__ int pflip_edge(uint64_t seed, int round, int B, int i, int ne)
@@ -298,42 +526,60 @@ __device__ __forceinline__ int pflip_edge(uint64_t seed, int round, int B, int i
     return rng_edge(seed ^ 0x5bd1e9955bd1e995ull, (uint64_t)round * (uint64_t)B + (uint64_t)i, ne);
 }
 
-__global__ void k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
-                              unsigned long long* __restrict__ owner)
+__device__ __forceinline__ unsigned long long pflip_key(int round, int B, int i)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nprop) return;
-    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
-    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
-    if (t.edge_boundary(e)) return;
-    unsigned long long key = ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
-    for_patch_edges(t, e, [&](int pe) { atomicMax(&owner[pe], key); });
+    return ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
 }
 
-// status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected
-__global__ void k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed, int round, int B, int nprop,
-                             const unsigned long long* __restrict__ owner, int* __restrict__ status,
-                             double* __restrict__ dprops)
+// four lanes per proposal, one per patch vertex (the maximum is the same in
+// any order of the atomics)
+__global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
+                                                    unsigned long long* __restrict__ owner)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.x * 16 + threadIdx.x / 4;
     if (i >= nprop) return;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
     int e = pflip_edge(seed, round, B, i, fm.nh / 2);
-    status[i] = 0;
     if (t.edge_boundary(e)) return;
-    unsigned long long key = ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
-    bool win = true;
-    for_patch_edges(t, e, [&](int pe) { win = win && owner[pe] == key; });
-    if (!win || !flip_ok(t, e)) return;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int s = 0; s < 4; s++) {
-        int h = 2 * e + (s >> 1);
-        int ve = (s & 1) ? t.to(t.next(h)) : t.to(h);
-        VP r = om_vertex_properties(t, nl, p, ve);
-        for (int k = 0; k < 6; k++) acc[k] += r.v[k];
+    const unsigned long long key = pflip_key(round, B, i);
+    for_patch_edges_of(t, e, threadIdx.x & 3, [&](int pe) { atomicMax(&owner[pe], key); });
+}
+
+// status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected.
+// One workgroup per proposal: lanes 0-3 check that the proposal's key
+// survived on their part of the patch while lane 4 tests is_flip_ok; a
+// winner's workgroup then evaluates the patch.
+__global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
+                                                   int round, int B, int nprop,
+                                                   const unsigned long long* __restrict__ owner,
+                                                   int* __restrict__ status, double* __restrict__ dprops)
+{
+    __shared__ CoopShared S;
+    __shared__ int s_go;
+    const int i = blockIdx.x;
+    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
+    const int e = pflip_edge(seed, round, B, i, fm.nh / 2);
+    if (threadIdx.x == 0) {
+        status[i] = 0;
+        s_go = 1;
     }
-    for (int k = 0; k < 6; k++) dprops[6 * i + k] = -acc[k];
-    status[i] = 1;
+    if (t.edge_boundary(e)) return;
+    __syncthreads();
+    const unsigned long long key = pflip_key(round, B, i);
+    if (threadIdx.x < 4) {
+        bool win = true;
+        for_patch_edges_of(t, e, threadIdx.x, [&](int pe) { win = win && owner[pe] == key; });
+        if (!win) s_go = 0;
+    } else if (threadIdx.x == 4) {
+        if (!flip_ok(t, e)) s_go = 0;
+    }
+    __syncthreads();
+    if (!s_go) return;
+    coop_patch(t, nl, p, e, S);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) dprops[6 * i + k] = -S.sum[k];
+        status[i] = 1;
+    }
 }
 
 __global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
@@ -344,51 +590,78 @@ __global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, 
     do_flip(fm, pflip_edge(seed, round, B, i, fm.nh / 2));
 }
 
-__global__ void k_pflip_eval2(FlipMeshView fm, NListView nl, EParams p, uint64_t seed, int round, int B, int nprop,
-                              const int* __restrict__ status, double* __restrict__ dprops)
+__global__ void __launch_bounds__(64) k_pflip_eval2(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
+                                                    int round, int B, int nprop, const int* __restrict__ status,
+                                                    double* __restrict__ dprops)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nprop || status[i] != 1) return;
+    __shared__ CoopShared S;
+    const int i = blockIdx.x;
+    if (status[i] != 1) return;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
-    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int s = 0; s < 4; s++) {
-        int h = 2 * e + (s >> 1);
-        int ve = (s & 1) ? t.to(t.next(h)) : t.to(h);
-        VP r = om_vertex_properties(t, nl, p, ve);
-        for (int k = 0; k < 6; k++) acc[k] += r.v[k];
-    }
-    for (int k = 0; k < 6; k++) dprops[6 * i + k] += acc[k];
+    coop_patch(t, nl, p, pflip_edge(seed, round, B, i, fm.nh / 2), S);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) dprops[6 * i + k] += S.sum[k];
 }
 
-// state: props[6], e0, accepted count (as double)
-__global__ void k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop, int* __restrict__ status,
-                             const double* __restrict__ dprops, double* __restrict__ state)
+// state: props[6], e0, accepted count (as double).  The workgroup stages a
+// chunk of statuses, deltas and uniforms in LDS; thread 0 then applies the
+// Metropolis test to the winners in proposal order.
+constexpr int kScanThreads = 256;
+constexpr int kScanChunk = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop,
+                                                             int* __restrict__ status,
+                                                             const double* __restrict__ dprops,
+                                                             double* __restrict__ state)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double props[6];
-    for (int k = 0; k < 6; k++) props[k] = state[k];
-    double e0 = state[6];
-    double acc = state[7];
-    for (int i = 0; i < nprop; i++) {
-        if (status[i] != 1) continue;
-        for (int k = 0; k < 6; k++) props[k] += dprops[6 * i + k];
-        double en = energy_of(props, p);
-        double de = en - e0;
-        double alpha = de < 0.0 ? 1.0 : exp(-de);
-        double u = rng_uniform(seed ^ 0x5bd1e9955bd1e995ull, (uint64_t)round * (uint64_t)B + (uint64_t)i);
-        if (u <= alpha) {
-            e0 = en;
-            acc += 1.0;
-            status[i] = 2;
-        } else {
-            for (int k = 0; k < 6; k++) props[k] -= dprops[6 * i + k];
-            status[i] = 3;
-        }
+    __shared__ int s_st[kScanChunk];
+    __shared__ double s_dp[6 * kScanChunk];
+    __shared__ double s_u[kScanChunk];
+    double props[6], e0 = 0.0, acc = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) props[k] = state[k];
+        e0 = state[6];
+        acc = state[7];
     }
-    for (int k = 0; k < 6; k++) state[k] = props[k];
-    state[6] = e0;
-    state[7] = acc;
+    for (int base = 0; base < nprop; base += kScanChunk) {
+        const int cnt = min(kScanChunk, nprop - base);
+        for (int k = threadIdx.x; k < cnt; k += kScanThreads) {
+            const int st = status[base + k];
+            s_st[k] = st;
+            if (st == 1)
+                s_u[k] = rng_uniform(seed ^ 0x5bd1e9955bd1e995ull,
+                                     (uint64_t)round * (uint64_t)B + (uint64_t)(base + k));
+        }
+        for (int k = threadIdx.x; k < 6 * cnt; k += kScanThreads) s_dp[k] = dprops[6 * (size_t)base + k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < cnt; k++) {
+                if (s_st[k] != 1) continue;
+                const double* dp = s_dp + 6 * k;
+                for (int c = 0; c < 6; c++) props[c] += dp[c];
+                double en = energy_of(props, p);
+                double de = en - e0;
+                double alpha = de < 0.0 ? 1.0 : exp(-de);
+                if (s_u[k] <= alpha) {
+                    e0 = en;
+                    acc += 1.0;
+                    s_st[k] = 2;
+                } else {
+                    for (int c = 0; c < 6; c++) props[c] -= dp[c];
+                    s_st[k] = 3;
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt; k += kScanThreads)
+            if (s_st[k] > 1) status[base + k] = s_st[k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) state[k] = props[k];
+        state[6] = e0;
+        state[7] = acc;
+    }
 }
 
 __global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* __restrict__ state)
@@ -409,11 +682,14 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
     for (int r = 0; r < rounds; r++) {
         int nprop = std::min(batch, nflips - r * batch);
         dim3 grid((nprop + 127) / 128), block(128);
-        hipLaunchKernelGGL(k_pflip_claim, grid, block, 0, s, fm, seed, r, batch, nprop, owner);
-        hipLaunchKernelGGL(k_pflip_eval, grid, block, 0, s, fm, nl, p, seed, r, batch, nprop, owner, status, dprops);
+        hipLaunchKernelGGL(k_pflip_claim, dim3((nprop + 15) / 16), dim3(64), 0, s, fm, seed, r, batch, nprop, owner);
+        hipLaunchKernelGGL(k_pflip_eval, dim3(nprop), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, owner, status,
+                           dprops);
         hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 1);
-        hipLaunchKernelGGL(k_pflip_eval2, grid, block, 0, s, fm, nl, p, seed, r, batch, nprop, status, dprops);
-        hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(64), 0, s, p, seed, r, batch, nprop, status, dprops, state);
+        hipLaunchKernelGGL(k_pflip_eval2, dim3(nprop), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, status,
+                           dprops);
+        hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop, status, dprops,
+                           state);
         hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 3);
     }
 }
