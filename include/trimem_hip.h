@@ -238,6 +238,10 @@ typedef struct tm_tuning {
     int pair_cache;      /* 1 (default): the cell-list query replays recorded
                             candidate lists while they are provably complete
                             (single meshes; bit-identical), 0: full query */
+    int pflip_window;    /* 1..256 (default 256): proposals the parallel flip
+                            sweep tests at once against the running state when
+                            it looks for the next acceptance; every window
+                            takes the decisions of the sequential walk (1) */
 } tm_tuning;
 int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t);
 int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out);

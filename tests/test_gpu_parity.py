@@ -327,6 +327,31 @@ def test_pflip_deterministic_and_manifold(core):
     assert sum(runs[0][0]) > 0
 
 
+def test_pflip_scan_window_takes_sequential_decisions(core):
+    """The parallel sweep's Metropolis scan tests `pflip_window` proposals at
+    once against the running state and applies the first acceptance; with a
+    window of 1 it is the sequential walk itself.  Level 5 (batches of 30
+    proposals) without repulsion accepts ~75% of the flips, so most batches
+    hold many acceptances: windows 1, 7 and 256 give the same accepted counts
+    and faces."""
+    v, f, p = flip_case(5, 25, False)
+    runs = {}
+    for w in (1, 7, 256):
+        gm = core.TriMesh(v, f)
+        ge = core.EnergyManager(gm, to_core_params(core, p))
+        t = ge.tuning
+        t["pflip_window"] = w
+        ge.tuning = t
+        assert ge.tuning["pflip_window"] == w
+        accs = [core.pflip(gm, ge, 0.2, seed=4321 + k) for k in range(3)]
+        runs[w] = (accs, gm.fv_indices().copy())
+        print(w, accs, flush=True)
+    assert min(runs[1][0]) > 0.2 * 0.2 * len(f) * 3 // 2
+    for w in (7, 256):
+        assert runs[w][0] == runs[1][0]
+        assert np.array_equal(runs[w][1], runs[1][1])
+
+
 def test_pflip_topology_consistent_with_oracle(core):
     """After parallel sweeps the device topology, ring table and host mirror
     agree: properties, energy and gradient match an oracle mesh rebuilt from

@@ -406,6 +406,7 @@ py::dict tuning_dict(const tm_tuning& t)
     d["pass2_threads"] = t.pass2_threads;
     d["rep_fused"] = t.rep_fused;
     d["pair_cache"] = t.pair_cache;
+    d["pflip_window"] = t.pflip_window;
     return d;
 }
 
@@ -417,6 +418,7 @@ tm_tuning tuning_of(const py::dict& d)
     t.pass2_threads = d["pass2_threads"].cast<int>();
     t.rep_fused = d.contains("rep_fused") ? d["rep_fused"].cast<int>() : 0;
     t.pair_cache = d.contains("pair_cache") ? d["pair_cache"].cast<int>() : 1;
+    t.pflip_window = d.contains("pflip_window") ? d["pflip_window"].cast<int>() : 256;
     return t;
 }
 

@@ -1220,7 +1220,8 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
     em->pf_dprops.alloc((size_t)6 * batch);
     em->pf_state.alloc(8);
     if (nflips > 0)
-        launch_flip_parallel(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips, batch, seed,
+        launch_flip_parallel(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips, batch,
+                             em->tune.pflip_window, seed,
                              em->globals.p, em->pf_owner.p, em->pf_status.p, em->pf_dprops.p, em->pf_state.p,
                              g_rt.stream);
     check_launch();
@@ -1796,6 +1797,7 @@ Tuning from_c(const tm_tuning& c)
     t.p2_threads = c.pass2_threads;
     t.rep_fused = c.rep_fused;
     t.pair_cache = c.pair_cache;
+    t.pflip_window = c.pflip_window;
     return t;
 }
 
@@ -1807,6 +1809,7 @@ tm_tuning to_c(const Tuning& t)
     c.pass2_threads = t.p2_threads;
     c.rep_fused = t.rep_fused;
     c.pair_cache = t.pair_cache;
+    c.pflip_window = t.pflip_window;
     return c;
 }
 

@@ -603,65 +603,74 @@ __global__ void __launch_bounds__(64) k_pflip_eval2(FlipMeshView fm, NListView n
         for (int k = 0; k < 6; k++) dprops[6 * i + k] += S.sum[k];
 }
 
-// state: props[6], e0, accepted count (as double).  The workgroup stages a
-// chunk of statuses, deltas and uniforms in LDS; thread 0 then applies the
-// Metropolis test to the winners in proposal order.
+// state: props[6], e0, accepted count (as double).  The Metropolis test
+// walks the winners in proposal order on the running properties
+// (flips.cpp:146-163), with a rejected flip restoring the running
+// properties exactly.  Most winners are rejected and a rejection leaves the
+// state unchanged, so the workgroup tests the next `window` proposals
+// against the current state at once, takes the first acceptance in
+// proposal order (an LDS atomicMin), applies it and continues after it:
+// every decision is the one the sequential walk makes, and the walk takes
+// one parallel step per acceptance instead of one dependent FP64 chain per
+// winner.  Statuses, deltas and uniforms are staged in LDS per chunk.
 constexpr int kScanThreads = 256;
 constexpr int kScanChunk = 1024;
 
 __global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop,
-                                                             int* __restrict__ status,
+                                                             int window, int* __restrict__ status,
                                                              const double* __restrict__ dprops,
                                                              double* __restrict__ state)
 {
     __shared__ int s_st[kScanChunk];
     __shared__ double s_dp[6 * kScanChunk];
     __shared__ double s_u[kScanChunk];
-    double props[6], e0 = 0.0, acc = 0.0;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) props[k] = state[k];
-        e0 = state[6];
-        acc = state[7];
-    }
+    __shared__ double s_state[8];
+    __shared__ int s_first[2];
+    const int tid = threadIdx.x;
+    if (tid < 8) s_state[tid] = state[tid];
     for (int base = 0; base < nprop; base += kScanChunk) {
         const int cnt = min(kScanChunk, nprop - base);
-        for (int k = threadIdx.x; k < cnt; k += kScanThreads) {
+        for (int k = tid; k < cnt; k += kScanThreads) {
             const int st = status[base + k];
             s_st[k] = st;
             if (st == 1)
                 s_u[k] = rng_uniform(seed ^ 0x5bd1e9955bd1e995ull,
                                      (uint64_t)round * (uint64_t)B + (uint64_t)(base + k));
         }
-        for (int k = threadIdx.x; k < 6 * cnt; k += kScanThreads) s_dp[k] = dprops[6 * (size_t)base + k];
+        for (int k = tid; k < 6 * cnt; k += kScanThreads) s_dp[k] = dprops[6 * (size_t)base + k];
+        if (tid == 0) s_first[0] = s_first[1] = cnt;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < cnt; k++) {
-                if (s_st[k] != 1) continue;
-                const double* dp = s_dp + 6 * k;
-                for (int c = 0; c < 6; c++) props[c] += dp[c];
-                double en = energy_of(props, p);
-                double de = en - e0;
-                double alpha = de < 0.0 ? 1.0 : exp(-de);
-                if (s_u[k] <= alpha) {
-                    e0 = en;
-                    acc += 1.0;
-                    s_st[k] = 2;
-                } else {
-                    for (int c = 0; c < 6; c++) props[c] -= dp[c];
-                    s_st[k] = 3;
-                }
+        // s_first[it & 1] collects iteration it's first acceptance; thread 0
+        // resets the other slot, whose readers finished before the last
+        // barrier and whose writers start after the next one
+        int it = 0;
+        for (int pos = 0; pos < cnt; it++) {
+            const int k = pos + tid;
+            double q[6], en = 0.0;
+            if (tid < window && k < cnt && s_st[k] == 1) {
+                for (int c = 0; c < 6; c++) q[c] = s_state[c] + s_dp[6 * k + c];
+                en = energy_of(q, p);
+                const double de = en - s_state[6];
+                const double alpha = de < 0.0 ? 1.0 : exp(-de);
+                if (s_u[k] <= alpha) atomicMin(&s_first[it & 1], k);
             }
+            if (tid == 0) s_first[(it + 1) & 1] = cnt;
+            __syncthreads();
+            const int kf = s_first[it & 1];
+            if (kf < cnt && k == kf) {
+                for (int c = 0; c < 6; c++) s_state[c] = q[c];
+                s_state[6] = en;
+                s_state[7] += 1.0;
+                s_st[k] = 2;
+            }
+            pos = kf < cnt ? kf + 1 : pos + window;
+            __syncthreads();
         }
-        __syncthreads();
-        for (int k = threadIdx.x; k < cnt; k += kScanThreads)
-            if (s_st[k] > 1) status[base + k] = s_st[k];
+        for (int k = tid; k < cnt; k += kScanThreads)
+            if (s_st[k] >= 1) status[base + k] = s_st[k] == 2 ? 2 : 3;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) state[k] = props[k];
-        state[6] = e0;
-        state[7] = acc;
-    }
+    if (tid < 8) state[tid] = s_state[tid];
 }
 
 __global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* __restrict__ state)
@@ -674,7 +683,7 @@ __global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* 
 }
 
 void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
-                          uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
+                          int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
                           double* state, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pflip_init, dim3(1), dim3(64), 0, s, g0, p, state);
@@ -688,8 +697,8 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
         hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 1);
         hipLaunchKernelGGL(k_pflip_eval2, dim3(nprop), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, status,
                            dprops);
-        hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop, status, dprops,
-                           state);
+        hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop,
+                           std::min(std::max(window, 1), kScanThreads), status, dprops, state);
         hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 3);
     }
 }

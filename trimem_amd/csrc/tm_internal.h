@@ -88,6 +88,7 @@ struct Tuning {
     int p2_threads = 256;     // 64, 128, 256 or 512
     int rep_fused = 0;        // 1: the query shares the pass-1 launch (256 threads), 0: its own launch
     int pair_cache = 1;       // 1: replay recorded candidate lists (single meshes, cell list)
+    int pflip_window = 256;   // proposals the pflip Metropolis scan tests at once (1..256)
 };
 bool tuning_valid(const Tuning& t);
 int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
@@ -185,8 +186,8 @@ void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EPara
 // graph-coloured (claim-based) parallel sweep; state receives props[6], the
 // final energy and the number of accepted flips
 void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
-                          uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
-                          double* state, hipStream_t s);
+                          int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status,
+                          double* dprops, double* state, hipStream_t s);
 
 // device HMC step (tm_hmc.hip)
 struct HmcState {
