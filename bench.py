@@ -104,6 +104,25 @@ def cpu_baseline(v, f, params, budget_s):
     return 1.0 / float(np.median(times)), len(times), O.num_threads()
 
 
+def cpu_flip_baseline(v, f, params, ratio=0.1, seed=7):
+    """The oracle's serial flip sweep (flips.cpp:22-80, one thread) on the
+    same mesh: one sweep of ratio * edges proposals, injected uniformly at
+    random with their uniforms; attempted flips per second."""
+    from oracle import oracle as O
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, O.Params(**params))
+    ne = len(f) * 3 // 2
+    n = int(ne * ratio)
+    rng = np.random.default_rng(seed)
+    prop = rng.integers(0, ne, n).astype(np.int32)
+    uni = rng.random(n)
+    t0 = time.perf_counter()
+    acc, used = oe.flip_serial_injected(om, prop, uni)
+    dt = time.perf_counter() - t0
+    return {"attempted_flips_per_s": n / dt, "ms_per_sweep": 1e3 * dt, "accepted": int(acc), "proposals": n,
+            "cores": 1}
+
+
 def core_params(core, params):
     """trimem_amd.core.EnergyParams of a workload() parameter dict."""
     e = core.EnergyParams()
@@ -710,6 +729,8 @@ def main():
                                "sample": f"{ns} FP64 energy()+gradient() evals of the same L{args.level} mesh, "
                                          f"median, OpenMP {thr} threads, {platform.processor() or 'host'}"}
         out["speedup_vs_cpu"] = evals_per_s / cv
+        if args.flips:
+            out["cpu_baseline"]["flips"] = cpu_flip_baseline(v, f, params)
     print(json.dumps(out), flush=True)
     L.tm_energy_destroy(em)
     L.tm_mesh_destroy(mesh)

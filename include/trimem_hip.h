@@ -299,6 +299,7 @@ int tm_replicas_stats(tm_replicas* rs, long long* accepted, long long* steps, in
 int tm_replicas_update_repulsion(tm_replicas* rs);
 /* launch shapes of the grouped launches (see tm_energy_autotune) */
 int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t);
+int tm_replicas_get_tuning(const tm_replicas* rs, tm_tuning* out);
 int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double* ms_pass2);
 
 /* ---- surrogate forces (no counterpart in the reference) ----

@@ -426,7 +426,6 @@ tm_tuning tuning_of(const py::dict& d)
 struct Replicas {
     tm_replicas* h = nullptr;
     std::vector<int> nv;
-    tm_tuning tune{256, 4, 256, 0};    // the library's default shape
     Replicas(const std::vector<TriMesh*>& meshes, const std::vector<EnergyManager*>& estores)
     {
         if (meshes.size() != estores.size()) throw std::runtime_error("one energy manager per mesh");
@@ -852,17 +851,20 @@ PYBIND11_MODULE(core, m)
              })
         .def("update_repulsion", [](Replicas& r) { check(tm_replicas_update_repulsion(r.h)); })
         .def_property(
-            "tuning", [](Replicas& r) { return tuning_dict(r.tune); },
+            "tuning",
+            [](Replicas& r) {
+                tm_tuning t{};
+                check(tm_replicas_get_tuning(r.h, &t));
+                return tuning_dict(t);
+            },
             [](Replicas& r, const py::dict& d) {
                 tm_tuning t = tuning_of(d);
                 check(tm_replicas_set_tuning(r.h, &t));
-                r.tune = t;
             })
         .def("autotune", [](Replicas& r) {
             tm_tuning out;
             double ms_rep = 0, ms_p2 = 0;
             check(tm_replicas_autotune(r.h, &out, &ms_rep, &ms_p2));
-            r.tune = out;
             py::dict d = tuning_dict(out);
             d["rep_ms"] = ms_rep;
             d["pass2_ms"] = ms_p2;

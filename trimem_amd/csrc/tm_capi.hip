@@ -1943,6 +1943,14 @@ int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t)
     API_END
 }
 
+int tm_replicas_get_tuning(const tm_replicas* rs, tm_tuning* out)
+{
+    API_BEGIN
+    require(out != nullptr, "null output");
+    *out = to_c(rs->tune);
+    API_END
+}
+
 int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double* ms_pass2)
 {
     API_BEGIN
