@@ -242,6 +242,10 @@ typedef struct tm_tuning {
                             sweep tests at once against the running state when
                             it looks for the next acceptance; every window
                             takes the decisions of the sequential walk (1) */
+    int serial_window;   /* 1..256 (default 256): proposals the serial flip sweep
+                            evaluates at once before committing them in order;
+                            0: one workgroup walks the proposals.  Every value
+                            gives the same accepted flips and topology */
 } tm_tuning;
 int tm_energy_set_tuning(tm_energy* em, const tm_tuning* t);
 int tm_energy_get_tuning(const tm_energy* em, tm_tuning* out);

@@ -269,7 +269,8 @@ def flip_case(level, seed, rep, nlist=0, excl=1):
 
 
 @pytest.mark.parametrize("level,rep,nlist,excl", [(2, False, 0, 1), (3, False, 0, 1), (3, True, 0, 1),
-                                                  (3, True, 0, 2), (3, True, 0, 0), (3, True, 1, 0), (3, True, 1, 1)])
+                                                  (3, True, 0, 2), (3, True, 0, 0), (3, True, 1, 0), (3, True, 1, 1),
+                                                  (5, True, 0, 2)])
 def test_flip_serial_injected_parity(core, level, rep, nlist, excl):
     """Accepted count, uniforms used and the face list after a serial sweep
     over an injected proposal/uniform stream equal the oracle's, for the
@@ -287,6 +288,64 @@ def test_flip_serial_injected_parity(core, level, rep, nlist, excl):
     # energy and gradient agree on the flipped topology
     assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
     assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+
+
+def with_window(core, gm, p, w):
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    t = ge.tuning
+    t["serial_window"] = w
+    ge.tuning = t
+    assert ge.tuning["serial_window"] == w
+    return ge
+
+
+@pytest.mark.parametrize("rep,excl", [(True, 2), (False, 1)])
+def test_serial_flip_windows_match_one_workgroup(core, rep, excl):
+    """The serial sweep in windows of W proposals (evaluated at once, then
+    committed in order up to the first proposal that an earlier flip of the
+    window could have changed) takes the one-workgroup sweep's decisions and
+    topology bit for bit.  Level 5 with ~73% acceptance, at exclusion level 2
+    (the neighbour-set rule) and without repulsion."""
+    v, f, p = flip_case(5, 26, rep, 0, excl)
+    runs = {}
+    for w in (0, 1, 7, 64, 256):
+        gm = core.TriMesh(v, f)
+        ge = with_window(core, gm, p, w)
+        accs = [core.flip(gm, ge, 0.3, seed=99 + k) for k in range(2)]
+        runs[w] = (accs, gm.fv_indices().copy(), ge.energy(gm))
+        print(w, accs, flush=True)
+    assert min(runs[0][0]) > 0.1 * len(f)
+    for w in (1, 7, 64, 256):
+        assert runs[w][0] == runs[0][0], w
+        assert np.array_equal(runs[w][1], runs[0][1]), w
+        assert runs[w][2] == runs[0][2], w
+
+
+def test_serial_flip_errors_match_one_workgroup(core):
+    """A proposal index out of range stops the sweep at that proposal, and an
+    exhausted uniform stream after the proposal that found it empty (which
+    counts as rejected): the windowed sweep raises the same error as the
+    one-workgroup sweep and leaves the same topology."""
+    v, f, p = flip_case(4, 27, False)
+    ne = len(f) * 3 // 2
+    rng = np.random.default_rng(5)
+    prop = rng.integers(0, ne, 2000).astype(np.int32)
+    bad = prop.copy()
+    bad[1500] = ne + 3
+    uni = rng.random(2000)
+    for args in ((bad, uni), (prop, uni[:700])):
+        outs = []
+        for w in (0, 256):
+            gm = core.TriMesh(v, f)
+            ge = with_window(core, gm, p, w)
+            with pytest.raises(RuntimeError) as ei:
+                core.flip_injected(gm, ge, *args)
+            outs.append((str(ei.value), gm.fv_indices().copy(), ge.energy(gm)))
+        print(outs[0][0], flush=True)
+        assert outs[0][0] == outs[1][0]
+        assert np.array_equal(outs[0][1], outs[1][1])
+        assert outs[0][2] == outs[1][2]
+        assert not np.array_equal(outs[0][1], f)
 
 
 def test_flip_smoke_reference_bound(core):

@@ -174,10 +174,14 @@ def test_exchange_rule(core):
     assert 0 < st2["swaps_accepted"].sum() < 9    # both outcomes exercised
 
 
-SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64, rep_fused=0, pair_cache=1, pflip_window=256),
-          dict(rep_threads=512, rep_batch=8, pass2_threads=512, rep_fused=1, pair_cache=0, pflip_window=1),
-          dict(rep_threads=128, rep_batch=2, pass2_threads=128, rep_fused=0, pair_cache=0, pflip_window=64),
-          dict(rep_threads=256, rep_batch=1, pass2_threads=256, rep_fused=1, pair_cache=1, pflip_window=256)]
+SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64, rep_fused=0, pair_cache=1, pflip_window=256,
+               serial_window=256),
+          dict(rep_threads=512, rep_batch=8, pass2_threads=512, rep_fused=1, pair_cache=0, pflip_window=1,
+               serial_window=0),
+          dict(rep_threads=128, rep_batch=2, pass2_threads=128, rep_fused=0, pair_cache=0, pflip_window=64,
+               serial_window=7),
+          dict(rep_threads=256, rep_batch=1, pass2_threads=256, rep_fused=1, pair_cache=1, pflip_window=256,
+               serial_window=128)]
 
 
 def vertex_properties(mesh, est):

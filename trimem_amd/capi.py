@@ -36,7 +36,7 @@ class tm_params(ctypes.Structure):
 
 class tm_tuning(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int) for k in
-                ("rep_threads", "rep_batch", "pass2_threads", "rep_fused", "pair_cache", "pflip_window")]
+                ("rep_threads", "rep_batch", "pass2_threads", "rep_fused", "pair_cache", "pflip_window", "serial_window")]
 
 
 class tm_props(ctypes.Structure):

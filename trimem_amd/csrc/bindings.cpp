@@ -407,6 +407,7 @@ py::dict tuning_dict(const tm_tuning& t)
     d["rep_fused"] = t.rep_fused;
     d["pair_cache"] = t.pair_cache;
     d["pflip_window"] = t.pflip_window;
+    d["serial_window"] = t.serial_window;
     return d;
 }
 
@@ -419,6 +420,7 @@ tm_tuning tuning_of(const py::dict& d)
     t.rep_fused = d.contains("rep_fused") ? d["rep_fused"].cast<int>() : 0;
     t.pair_cache = d.contains("pair_cache") ? d["pair_cache"].cast<int>() : 1;
     t.pflip_window = d.contains("pflip_window") ? d["pflip_window"].cast<int>() : 256;
+    t.serial_window = d.contains("serial_window") ? d["serial_window"].cast<int>() : 256;
     return t;
 }
 

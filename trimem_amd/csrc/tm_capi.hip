@@ -584,6 +584,10 @@ struct tm_energy {
     DevBuf<int> flip_result, proposals, pf_status;
     DevBuf<double> uniforms, pf_dprops, pf_state;
     DevBuf<unsigned long long> pf_owner;
+    DevBuf<SerialState> sf_state;
+    DevBuf<int> sf_int;
+    DevBuf<double> sf_dbl;
+    DevBuf<unsigned long long> sf_vstamp, sf_estamp, sf_fstamp;
     Tuning tune;
     tm_nlist* spare = nullptr;    // the previous list, rebuilt in place by update_repulsion
     ~tm_energy()
@@ -1162,6 +1166,51 @@ int tm_nlist_point_distances(tm_nlist* nl, tm_mesh* mesh, int pid, double dmax, 
 }
 
 // ---- flips ----
+// The serial sweep in windows of W proposals (launch_serial_windows).  The
+// number of windows a sweep needs depends on how many proposals each
+// commits, so they are launched in batches sized from the rate so far, with
+// the state read back between batches; surplus windows return at once.
+static void serial_windows(tm_energy& E, tm_mesh& M, int nflips, const int* proposals, const double* uniforms,
+                           int n_uniforms, uint64_t seed, int injected, int W)
+{
+    const size_t w = (size_t)W;
+    E.sf_state.alloc(1);
+    E.sf_int.alloc(w * (2 + 2 + 4 + 4 + 4 * kMaxValence));
+    E.sf_dbl.alloc(w * 12);
+    E.sf_vstamp.alloc(M.nv);
+    E.sf_estamp.alloc(M.nh / 2);
+    HIPCHECK(hipMemsetAsync(E.sf_vstamp.p, 0, sizeof(unsigned long long) * M.nv, g_rt.stream));
+    HIPCHECK(hipMemsetAsync(E.sf_estamp.p, 0, sizeof(unsigned long long) * (M.nh / 2), g_rt.stream));
+    E.sf_fstamp.alloc(M.nf);
+    HIPCHECK(hipMemsetAsync(E.sf_fstamp.p, 0, sizeof(unsigned long long) * M.nf, g_rt.stream));
+    SerialWindow sw;
+    sw.e = E.sf_int.p;
+    sw.valid = sw.e + w;
+    sw.pv = sw.valid + w;
+    sw.fc = sw.pv + 4 * w;
+    sw.nbn = sw.fc + 2 * w;
+    sw.nb = sw.nbn + 4 * w;
+    sw.dO = E.sf_dbl.p;
+    sw.dN = sw.dO + 6 * w;
+    const EParams p = eparams_of(E);
+    const NListView nl = E.nlist->view();
+    launch_serial_init(E.globals.p, p, nflips, E.sf_state.p, g_rt.stream);
+    int batch = std::max(1, std::min(1024, nflips / W + 1));
+    SerialState h{};
+    for (;;) {
+        launch_serial_windows(M.flip_view(), nl, p, nflips, proposals, uniforms, n_uniforms, seed, injected, W,
+                              E.sf_state.p, sw, E.sf_vstamp.p, E.sf_estamp.p, E.sf_fstamp.p, batch, g_rt.stream);
+        check_launch();
+        d2h(&h, E.sf_state.p, sizeof(SerialState));
+        sync();
+        if (h.done) break;
+        const double per_window = std::max(1.0, (double)h.pos / std::max(h.window, 1));
+        batch = std::min(4096, (int)std::ceil((nflips - h.pos) / per_window * 1.1) + 1);
+    }
+    int res[3] = {h.acc, h.used, h.err};
+    h2d(E.flip_result.p, res, sizeof(res));
+}
+
 static int flip_impl(tm_energy* em, tm_mesh* mesh, int nflips, const int* proposals, const double* uniforms,
                      int n_uniforms, uint64_t seed, int injected, int* accepted, int* n_used)
 {
@@ -1176,10 +1225,16 @@ static int flip_impl(tm_energy* em, tm_mesh* mesh, int nflips, const int* propos
         em->uniforms.alloc(std::max(n_uniforms, 1));
         if (n_uniforms > 0) h2d(em->uniforms.p, uniforms, sizeof(double) * n_uniforms);
     }
-    launch_flip_serial(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips,
-                       injected ? em->proposals.p : nullptr, injected ? em->uniforms.p : nullptr, n_uniforms, seed,
-                       injected, em->globals.p, em->flip_result.p, g_rt.stream);
-    check_launch();
+    const int W = em->tune.serial_window;
+    if (W == 0) {
+        launch_flip_serial(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips,
+                           injected ? em->proposals.p : nullptr, injected ? em->uniforms.p : nullptr, n_uniforms,
+                           seed, injected, em->globals.p, em->flip_result.p, g_rt.stream);
+        check_launch();
+    } else {
+        serial_windows(*em, *mesh, nflips, injected ? em->proposals.p : nullptr, injected ? em->uniforms.p : nullptr,
+                       n_uniforms, seed, injected, W);
+    }
     HIPCHECK(hipMemsetAsync(mesh->flag.p, 0, sizeof(int), g_rt.stream));
     launch_build_ring(mesh->view(), mesh->ring.p, mesh->meta.p, mesh->flag.p, g_rt.stream);
     check_launch();
@@ -1798,6 +1853,7 @@ Tuning from_c(const tm_tuning& c)
     t.rep_fused = c.rep_fused;
     t.pair_cache = c.pair_cache;
     t.pflip_window = c.pflip_window;
+    t.serial_window = c.serial_window;
     return t;
 }
 
@@ -1810,6 +1866,7 @@ tm_tuning to_c(const Tuning& t)
     c.rep_fused = t.rep_fused;
     c.pair_cache = t.pair_cache;
     c.pflip_window = t.pflip_window;
+    c.serial_window = t.serial_window;
     return c;
 }
 

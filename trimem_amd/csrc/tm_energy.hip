@@ -1348,7 +1348,7 @@ bool tuning_valid(const Tuning& t)
     return th(t.rep_threads) && th(t.p2_threads) &&
            (t.rep_batch == 1 || t.rep_batch == 2 || t.rep_batch == 4 || t.rep_batch == 8) &&
            (t.rep_fused == 0 || t.rep_fused == 1) && (t.pair_cache == 0 || t.pair_cache == 1) &&
-           t.pflip_window >= 1 && t.pflip_window <= 256;
+           t.pflip_window >= 1 && t.pflip_window <= 256 && t.serial_window >= 0 && t.serial_window <= 256;
 }
 
 int tuned_blocks(int nv, int threads) { return nblocks(nblocks(nv * kGroup, threads), kXcds) * kXcds; }

@@ -84,7 +84,7 @@ __device__ __forceinline__ bool flip_ok(const Topo& t, int e)
     int ah = t.to(t.next(2 * e));
     int bh = t.to(t.next(2 * e + 1));
     if (ah == bh) return false;
-    int h0 = t.v_he[ah];
+    int h0 = t.vhe(ah);
     int h = h0;
     int guard = 0;
     do {
@@ -117,6 +117,98 @@ __device__ __forceinline__ void do_flip(const FlipMeshView& fm, int e)
     fm.f_he[fb] = b0;
     if (fm.v_he[va0] == b0) fm.v_he[va0] = a1;
     if (fm.v_he[vb0] == a0) fm.v_he[vb0] = b1;
+}
+
+// The topology after flipping interior edge e, without writing it: every
+// accessor returns what do_flip stores, so a workgroup can evaluate the
+// flipped patch while others read the unflipped mesh.
+struct FlippedTopo {
+    Topo t;
+    const double* x;
+    int a0, a1, a2, b0, b1, b2;
+    int va1, vb1, va0, vb0, fa, fb;
+    int vhe_a, vhe_b;    // v_he of va0 and vb0 after the flip
+
+    __device__ FlippedTopo(const Topo& base, int e) : t(base), x(base.x)
+    {
+        a0 = 2 * e;
+        b0 = 2 * e + 1;
+        a1 = t.next(a0);
+        a2 = t.next(a1);
+        b1 = t.next(b0);
+        b2 = t.next(b1);
+        va0 = t.to(a0);
+        va1 = t.to(a1);
+        vb0 = t.to(b0);
+        vb1 = t.to(b1);
+        fa = t.face(a0);
+        fb = t.face(b0);
+        const int ha = t.vhe(va0), hb = t.vhe(vb0);
+        vhe_a = ha == b0 ? a1 : ha;
+        vhe_b = hb == a0 ? b1 : hb;
+    }
+    __device__ __forceinline__ d3 P(int v) const { return t.P(v); }
+    __device__ __forceinline__ int to(int h) const { return h == a0 ? va1 : (h == b0 ? vb1 : t.to(h)); }
+    __device__ __forceinline__ int from(int h) const { return to(h ^ 1); }
+    __device__ __forceinline__ int next(int h) const
+    {
+        if (h == a0) return a2;
+        if (h == a2) return b1;
+        if (h == b1) return a0;
+        if (h == b0) return b2;
+        if (h == b2) return a1;
+        if (h == a1) return b0;
+        return t.next(h);
+    }
+    __device__ __forceinline__ int prev(int h) const
+    {
+        if (h == a2) return a0;
+        if (h == b1) return a2;
+        if (h == a0) return b1;
+        if (h == b2) return b0;
+        if (h == a1) return b2;
+        if (h == b0) return a1;
+        return t.prev(h);
+    }
+    __device__ __forceinline__ int face(int h) const { return h == a1 ? fb : (h == b1 ? fa : t.face(h)); }
+    __device__ __forceinline__ bool he_boundary(int h) const { return face(h) < 0; }
+    __device__ __forceinline__ bool edge_boundary(int e) const { return face(2 * e) < 0 || face(2 * e + 1) < 0; }
+    __device__ __forceinline__ int cw(int h) const { return next(h ^ 1); }
+    __device__ __forceinline__ int vhe(int v) const { return v == va0 ? vhe_a : (v == vb0 ? vhe_b : t.vhe(v)); }
+    __device__ __forceinline__ int fhe(int f) const { return f == fa ? a0 : (f == fb ? b0 : t.fhe(f)); }
+};
+
+// do_flip(fm, e) twice (a rejected proposal), stored in one pass.  The
+// second flip undoes the connectivity but not the ids: the edge's two
+// halfedges swap direction, the two faces swap triangles, and the v_he of
+// the edge's endpoints move off the swapped halfedges exactly as the first
+// flip moved them (the second flip's v_he tests cannot hold, since its
+// endpoints' outgoing halfedges are not the edge's).
+__device__ __forceinline__ void do_flip_twice(const FlipMeshView& fm, int e)
+{
+    int a0 = 2 * e, b0 = 2 * e + 1;
+    int a1 = fm.he_next[a0], a2 = fm.he_next[a1];
+    int b1 = fm.he_next[b0], b2 = fm.he_next[b1];
+    int va0 = fm.he_to[a0], vb0 = fm.he_to[b0];
+    int ta2 = fm.he_to[a2], tb2 = fm.he_to[b2];
+    int fa = fm.he_face[a0], fb = fm.he_face[b0];
+    int ha = fm.v_he[va0], hb = fm.v_he[vb0];
+    fm.he_to[a0] = ta2;
+    fm.he_to[b0] = tb2;
+    fm.he_next[a0] = b1; fm.he_prev[b1] = a0;
+    fm.he_next[b1] = b2; fm.he_prev[b2] = b1;
+    fm.he_next[b2] = a0; fm.he_prev[a0] = b2;
+    fm.he_next[b0] = a1; fm.he_prev[a1] = b0;
+    fm.he_next[a1] = a2; fm.he_prev[a2] = a1;
+    fm.he_next[a2] = b0; fm.he_prev[b0] = a2;
+    fm.he_face[a1] = fb;
+    fm.he_face[a2] = fb;
+    fm.he_face[b1] = fa;
+    fm.he_face[b2] = fa;
+    fm.f_he[fa] = a0;
+    fm.f_he[fb] = b0;
+    if (ha == b0) fm.v_he[va0] = a1;
+    if (hb == a0) fm.v_he[vb0] = b1;
 }
 
 __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p)
@@ -171,7 +263,8 @@ struct CoopShared {
     double sum[6];
 };
 
-__device__ __forceinline__ int patch_vertex(const Topo& t, int e, int g)
+template <class TP>
+__device__ __forceinline__ int patch_vertex(const TP& t, int e, int g)
 {
     int h = 2 * e + (g >> 1);
     return (g & 1) ? t.to(t.next(h)) : t.to(h);
@@ -243,7 +336,8 @@ __device__ __forceinline__ bool coop_excluded(const CoopShared& S, int g, int n,
 
 // Called by all 64 threads of the workgroup; S.vp and S.sum hold the result
 // on return.
-__device__ void coop_patch(const Topo& t, const NListView& nl, const EParams& p, int e, CoopShared& S)
+template <class TP>
+__device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, int e, CoopShared& S)
 {
     const int g = threadIdx.x / kCoopLanes, l = threadIdx.x % kCoopLanes;
     const int ve = patch_vertex(t, e, g);
@@ -255,7 +349,7 @@ __device__ void coop_patch(const Topo& t, const NListView& nl, const EParams& p,
     CandRuns runs = cand_runs(nl, ve, pi);
     if (l == 0) {
         int n = 0;
-        int h0 = t.v_he[ve];
+        int h0 = t.vhe(ve);
         if (h0 >= 0) {
             int h = h0;
             do {
@@ -290,7 +384,7 @@ __device__ void coop_patch(const Topo& t, const NListView& nl, const EParams& p,
             S.slot[g][s][3] = p.bond_type == 0 ? bond_edge_value(el, p) : bond_area_value(mu_face_area(t, h), p);
         }
         if (rep_on && excl > 1) {
-            const int g0 = t.v_he[S.nb[g][s]];
+            const int g0 = t.vhe(S.nb[g][s]);
             int m = 0;
             if (g0 >= 0) {
                 int gh = g0;
@@ -482,6 +576,476 @@ void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EPara
                        seed, injected, g0, result);
 }
 
+// ================================================ windowed serial sweep ==
+// The serial sweep's decisions and topology, bit for bit, from windows of W
+// proposals evaluated at once.  Per window:
+//   k_sf_eval    one workgroup per proposal and side: is_flip_ok and the
+//                patch properties before the flip (side 0) and after it on
+//                FlippedTopo (side 1), on the topology at the window start.
+//   k_sf_commit  one workgroup commits the window in order up to the first
+//                proposal whose evaluation an earlier one may have changed.
+// What an earlier proposal j of the window changes for a later one k:
+//   * accepted: the one-rings of j's four patch vertices V(j).  k read the
+//     one-rings of its patch vertices P_k (in circulator order) and, for
+//     exclusion level 2, the neighbour sets of their one-ring vertices
+//     N(P_k); so V(j) meeting P_k (or N(P_k)) makes k stale.
+//   * rejected (do_flip twice): every vertex keeps its neighbours in the
+//     same cyclic order, and each halfedge a walk reaches leads to the same
+//     vertex; only ids move.  The edge's two halfedges swap direction (a
+//     later proposal of the same edge sees its patch in another order), and
+//     an endpoint whose v_he was one of them starts its circulator
+//     elsewhere (a later proposal with that vertex in P_k sums its ring in
+//     another order).  The faces' f_he move too; only boundary sector
+//     angles read them, and no flip evaluation does.
+// Valid proposals stamp their edge and those endpoints with their window
+// position (atomicMax of a window-keyed 64-bit key keeps the smallest);
+// accepted ones are checked against the later proposals' P_k and N(P_k) as
+// they are found.  A rejected proposal can still share a triangle with a
+// later committed one, so flips whose faces an earlier valid proposal of the
+// window touched (stamped like the edges) are applied after the others, in
+// order, by one lane.  Up to the first stale proposal, lanes 0-5 run the serial
+// sweep's rounding path (props -= old, += new; on rejection -= new, += old)
+// through the rejected branch, one component each; the lanes then take the
+// Metropolis decisions at once, and the first acceptance ends the segment:
+// its state is the serial one, and the walk continues after it.
+// A window commits at least its first proposal, which cannot be stale.
+__device__ __forceinline__ unsigned long long sf_key(int window, int W, int k)
+{
+    return ((unsigned long long)(unsigned)(window + 1) << 32) | (unsigned)(W - k);
+}
+
+// window position recorded in a stamp, or W if the stamp is older
+__device__ __forceinline__ int sf_pos(unsigned long long key, unsigned wid, int W)
+{
+    return (unsigned)(key >> 32) == wid ? W - (int)(unsigned)(key & 0xffffffffu) : W;
+}
+
+__global__ void __launch_bounds__(64) k_sf_init(const Globals* __restrict__ g0, EParams p, int nflips,
+                                                SerialState* __restrict__ st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double props[6];
+    for (int k = 0; k < 6; k++) props[k] = st->props[k] = g0->props[k];
+    st->e0 = energy_of(props, p);
+    st->pos = 0;
+    st->done = nflips <= 0;
+    st->used = st->acc = st->err = st->window = 0;
+}
+
+__global__ void __launch_bounds__(64) k_sf_eval(FlipMeshView fm, NListView nl, EParams p, int nflips,
+                                                const int* __restrict__ proposals, uint64_t seed, int injected,
+                                                int W, int need_nb, const SerialState* __restrict__ st,
+                                                SerialWindow w, unsigned long long* __restrict__ vstamp,
+                                                unsigned long long* __restrict__ estamp,
+                                                unsigned long long* __restrict__ fstamp)
+{
+    __shared__ CoopShared S;
+    __shared__ int s_ok;
+    if (st->done) return;
+    const int k = blockIdx.x, side = blockIdx.y, tid = threadIdx.x;
+    const int i = st->pos + k;
+    if (i >= nflips) return;
+    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
+    const int ne = fm.nh / 2;
+    const int e = injected ? proposals[i] : rng_edge(seed, (uint64_t)i, ne);
+    if (e < 0 || e >= ne) {
+        if (side == 0 && tid == 0) w.valid[k] = -1;
+        return;
+    }
+    // the vertices whose one-rings decide validity and the patch terms (for
+    // a boundary edge, is_flip_ok reads only the edge's own halfedges)
+    if (side == 0 && tid < 4)
+        w.pv[4 * k + tid] = t.edge_boundary(e) ? t.to(2 * e + (tid & 1)) : patch_vertex(t, e, tid);
+    if (side == 0 && tid == 0) w.e[k] = e;
+    if (tid == 0) s_ok = flip_ok(t, e);
+    __syncthreads();
+    if (!s_ok) {
+        if (side == 0 && tid == 0) w.valid[k] = 0;
+        return;
+    }
+    if (side == 0) {
+        coop_patch(t, nl, p, e, S);
+        const unsigned long long key = sf_key(st->window, W, k);
+        if (tid == 0) {
+            for (int c = 0; c < 6; c++) w.dO[6 * k + c] = S.sum[c];
+            w.valid[k] = 1;
+            atomicMax(&estamp[e], key);
+        } else if (tid >= 3 && tid < 5) {
+            const int f = t.face(2 * e + tid - 3);
+            w.fc[2 * k + tid - 3] = f;
+            atomicMax(&fstamp[f], key);
+        } else if (tid < 3) {
+            // an endpoint whose v_he is one of the edge's halfedges
+            const int h = 2 * e + tid - 1;
+            const int v = t.to(h);
+            if (t.vhe(v) == (h ^ 1)) atomicMax(&vstamp[v], key);
+        }
+        if (need_nb) {
+            const int g = tid / kCoopLanes, l = tid % kCoopLanes;
+            const int n = S.n[g];
+            if (l == 0) w.nbn[4 * k + g] = n;
+            for (int s = l; s < n; s += kCoopLanes) w.nb[(size_t)(4 * k + g) * kMaxValence + s] = S.nb[g][s];
+        }
+    } else {
+        const FlippedTopo ft(t, e);
+        coop_patch(ft, nl, p, e, S);
+        if (tid == 0)
+            for (int c = 0; c < 6; c++) w.dN[6 * k + c] = S.sum[c];
+    }
+}
+
+// The reads of do_flip for up to kPer edges, then the writes: the edges'
+// halfedges are disjoint, so the loads of all of them overlap.
+template <int N>
+__device__ __forceinline__ void flip_batch(const FlipMeshView& fm, const int (&e)[N], const int (&mode)[N])
+{
+    // mode: 0 none, 1 do_flip, 2 do_flip twice
+    int a1[N], a2[N], b1[N], b2[N], va0[N], vb0[N], va1[N], vb1[N], ta2[N], tb2[N], fa[N], fb[N], ha[N], hb[N];
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        if (!mode[r]) continue;
+        const int a0 = 2 * e[r], b0 = a0 + 1;
+        a1[r] = fm.he_next[a0];
+        b1[r] = fm.he_next[b0];
+        va0[r] = fm.he_to[a0];
+        vb0[r] = fm.he_to[b0];
+        fa[r] = fm.he_face[a0];
+        fb[r] = fm.he_face[b0];
+    }
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        if (!mode[r]) continue;
+        a2[r] = fm.he_next[a1[r]];
+        b2[r] = fm.he_next[b1[r]];
+        va1[r] = fm.he_to[a1[r]];
+        vb1[r] = fm.he_to[b1[r]];
+        ha[r] = fm.v_he[va0[r]];
+        hb[r] = fm.v_he[vb0[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        if (!mode[r]) continue;
+        ta2[r] = fm.he_to[a2[r]];
+        tb2[r] = fm.he_to[b2[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        if (!mode[r]) continue;
+        const int a0 = 2 * e[r], b0 = a0 + 1;
+        if (mode[r] == 1) {
+            // do_flip
+            fm.he_to[a0] = va1[r];
+            fm.he_to[b0] = vb1[r];
+            fm.he_next[a0] = a2[r]; fm.he_prev[a2[r]] = a0;
+            fm.he_next[a2[r]] = b1[r]; fm.he_prev[b1[r]] = a2[r];
+            fm.he_next[b1[r]] = a0; fm.he_prev[a0] = b1[r];
+            fm.he_next[b0] = b2[r]; fm.he_prev[b2[r]] = b0;
+            fm.he_next[b2[r]] = a1[r]; fm.he_prev[a1[r]] = b2[r];
+            fm.he_next[a1[r]] = b0; fm.he_prev[b0] = a1[r];
+            fm.he_face[a1[r]] = fb[r];
+            fm.he_face[b1[r]] = fa[r];
+        } else {
+            // do_flip_twice
+            fm.he_to[a0] = ta2[r];
+            fm.he_to[b0] = tb2[r];
+            fm.he_next[a0] = b1[r]; fm.he_prev[b1[r]] = a0;
+            fm.he_next[b1[r]] = b2[r]; fm.he_prev[b2[r]] = b1[r];
+            fm.he_next[b2[r]] = a0; fm.he_prev[a0] = b2[r];
+            fm.he_next[b0] = a1[r]; fm.he_prev[a1[r]] = b0;
+            fm.he_next[a1[r]] = a2[r]; fm.he_prev[a2[r]] = a1[r];
+            fm.he_next[a2[r]] = b0; fm.he_prev[b0] = a2[r];
+            fm.he_face[a1[r]] = fb[r];
+            fm.he_face[a2[r]] = fb[r];
+            fm.he_face[b1[r]] = fa[r];
+            fm.he_face[b2[r]] = fa[r];
+        }
+        fm.f_he[fa[r]] = a0;
+        fm.f_he[fb[r]] = b0;
+        if (ha[r] == b0) fm.v_he[va0[r]] = a1[r];
+        if (hb[r] == a0) fm.v_he[vb0[r]] = b1[r];
+    }
+}
+
+__global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, int nflips,
+                                                  const double* __restrict__ uniforms, int n_uniforms,
+                                                  uint64_t seed, int injected, int W, int need_nb,
+                                                  SerialState* __restrict__ st, SerialWindow w,
+                                                  const unsigned long long* __restrict__ vstamp,
+                                                  const unsigned long long* __restrict__ estamp,
+                                                  const unsigned long long* __restrict__ fstamp)
+{
+    constexpr int kPer = kSerialWindowMax / 64;
+    constexpr int kChunk = 64;                  // valid proposals per decision round
+    __shared__ int s_valid[kSerialWindowMax];
+    __shared__ int s_stale[kSerialWindowMax];
+    __shared__ int s_defer[kSerialWindowMax];   // an earlier valid proposal has one of its faces
+    __shared__ int s_accf[kSerialWindowMax];    // accepted
+    __shared__ int s_vk[kSerialWindowMax];      // window position of the m-th valid proposal
+    __shared__ int s_rank[kSerialWindowMax + 1];    // valid proposals before position k
+    __shared__ int s_pv[4 * kSerialWindowMax];
+    __shared__ int s_nbn[4 * kSerialWindowMax];
+    __shared__ int s_ring[4 * kMaxValence];     // one-rings of an accepted proposal's patch
+    __shared__ int s_dlist[kSerialWindowMax];   // positions of the deferred flips, ascending
+    __shared__ unsigned s_bits[32];             // hash bitmap of s_ring and the accepted patch
+    // by rank among the valid proposals: uniforms, the rounding path's
+    // states and the patch properties before and after the flip
+    __shared__ double s_u[kSerialWindowMax];
+    __shared__ double s_q[6 * kSerialWindowMax];
+    __shared__ double s_o[6 * kSerialWindowMax];
+    __shared__ double s_n[6 * kSerialWindowMax];
+    __shared__ double s_P[6], s_Prej[6];
+    __shared__ double s_e0;
+    __shared__ int s_acc, s_err, s_end, s_first, s_ovf, s_ndef;
+    const int tid = threadIdx.x;
+    if (st->done) return;
+    const int pos0 = st->pos;
+    const int cnt = min(W, nflips - pos0);
+    const unsigned wid = (unsigned)st->window + 1u;
+    const int used0 = st->used;
+
+    // ---- stage; position k = 64 r + tid
+    int v[kPer], e[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = 64 * r + tid;
+        v[r] = k < cnt ? w.valid[k] : -2;
+        e[r] = v[r] >= 0 ? w.e[k] : 0;
+    }
+    int pv[kPer][4], fc[kPer][2];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = 64 * r + tid;
+        for (int g = 0; g < 4; g++) pv[r][g] = v[r] >= 0 ? w.pv[4 * k + g] : 0;
+        for (int g = 0; g < 2; g++) fc[r][g] = v[r] == 1 ? w.fc[2 * k + g] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = 64 * r + tid;
+        int stale = 0, defer = 0;
+        if (v[r] >= 0) {
+            for (int g = 0; g < 4; g++) stale |= sf_pos(vstamp[pv[r][g]], wid, W) < k;
+            stale |= sf_pos(estamp[e[r]], wid, W) < k;
+            for (int g = 0; g < 4; g++) s_pv[4 * k + g] = pv[r][g];
+        }
+        if (v[r] == 1) {
+            defer = sf_pos(fstamp[fc[r][0]], wid, W) < k || sf_pos(fstamp[fc[r][1]], wid, W) < k;
+            if (need_nb)
+                for (int g = 0; g < 4; g++) s_nbn[4 * k + g] = w.nbn[4 * k + g];
+        }
+        s_valid[k] = v[r];
+        s_stale[k] = stale;
+        s_defer[k] = defer;
+        s_accf[k] = 0;
+    }
+    // ranks among the valid proposals (their uniform index is used0 + rank)
+    // and the list of deferred flips
+    int base = 0, dbase = 0;
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = 64 * r + tid;
+        const unsigned long long below = (1ull << tid) - 1ull;
+        const unsigned long long bd = __ballot(s_defer[k] != 0);
+        if (s_defer[k]) s_dlist[dbase + __popcll(bd & below)] = k;
+        dbase += __popcll(bd);
+        const unsigned long long b = __ballot(v[r] == 1);
+        const int m = base + __popcll(b & ((1ull << tid) - 1ull));
+        s_rank[k] = m;
+        if (v[r] == 1) {
+            const int u = used0 + m;
+            s_vk[m] = k;
+            s_u[m] = injected ? (u < n_uniforms ? uniforms[u] : 2.0) : rng_uniform(seed, (uint64_t)u);
+            for (int c = 0; c < 6; c++) {
+                s_o[6 * m + c] = w.dO[6 * k + c];
+                s_n[6 * m + c] = w.dN[6 * k + c];
+            }
+        }
+        base += __popcll(b);
+    }
+    if (tid == 0) {
+        s_rank[kSerialWindowMax] = base;
+        s_ndef = dbase;
+        for (int c = 0; c < 6; c++) s_P[c] = st->props[c];
+        s_e0 = st->e0;
+        s_acc = st->acc;
+        s_err = 0;
+        // an injected stream too short: the proposal that finds it empty is
+        // rejected (u = 2) and the sweep stops after it
+        s_ovf = (injected && base > n_uniforms - used0) ? max(n_uniforms - used0, 0) : kSerialWindowMax;
+    }
+    __syncthreads();
+    const int movf = s_ovf;    // rank of that proposal
+    const int ovf = movf < base ? s_vk[movf] : cnt;
+
+    // ---- decide, chunk by chunk, up to the first stale proposal
+    int pos = 0;
+    while (pos < cnt) {
+        if (tid == 0) s_end = cnt;
+        __syncthreads();
+        for (int k = pos + tid; k < cnt; k += 64)
+            if (s_stale[k] || s_valid[k] < 0) atomicMin(&s_end, k);
+        __syncthreads();
+        int end = s_end;
+        if (end == pos) break;
+        if (ovf < end) end = ovf + 1;
+        const int m0 = s_rank[pos], mend = s_rank[end];
+        const int m1 = min(mend, m0 + kChunk);
+        const int cend = m1 < mend ? s_vk[m1] : end;    // positions [pos, cend) hold valid m0 .. m1-1
+        if (tid < 6) {
+            // the serial rounding path through the rejected branch, one lane
+            // per component
+            const int c = tid;
+            double P = s_P[c];
+#pragma unroll 4
+            for (int m = m0; m < m1; m++) {
+                const double o = s_o[6 * m + c], nw = s_n[6 * m + c];
+                double q = P - o;
+                q += nw;
+                s_q[6 * m + c] = q;
+                P = q - nw;
+                P += o;
+            }
+            s_Prej[c] = P;
+        }
+        if (tid == 0) s_first = cend;
+        __syncthreads();
+        const double e0 = s_e0;
+        for (int m = m0 + tid; m < m1; m += 64) {
+            const int k = s_vk[m];
+            if (k == ovf) continue;
+            const double en = energy_of(s_q + 6 * m, p);
+            const double de = en - e0;
+            const double alpha = de < 0.0 ? 1.0 : exp(-de);
+            if (s_u[m] <= alpha) atomicMin(&s_first, k);
+        }
+        __syncthreads();
+        const int kf = s_first;
+        if (kf < cend) {
+            if (tid == 0) {
+                const int mf = s_rank[kf];
+                for (int c = 0; c < 6; c++) s_P[c] = s_q[6 * mf + c];
+                s_e0 = energy_of(s_q + 6 * mf, p);
+                s_acc += 1;
+                s_accf[kf] = 1;
+            }
+            // later proposals whose patch meets the accepted patch are stale;
+            // for exclusion level 2 also those with a patch vertex adjacent
+            // to it (adjacency is symmetric: the accepted patch meets their
+            // patch's one-rings), found in the accepted patch's one-rings
+            const int f0 = s_pv[4 * kf], f1 = s_pv[4 * kf + 1], f2 = s_pv[4 * kf + 2], f3 = s_pv[4 * kf + 3];
+            auto in_v = [&](int u) { return u == f0 || u == f1 || u == f2 || u == f3; };
+            auto hbit = [](int u) { return ((unsigned)u * 2654435761u) >> 22; };    // 10 bits
+            int rn[4] = {0, 0, 0, 0};
+            bool all = false;
+            if (tid < 32) s_bits[tid] = 0u;
+            if (need_nb) {
+                for (int g = 0; g < 4; g++) {
+                    rn[g] = s_nbn[4 * kf + g];
+                    all = all || rn[g] < 0;
+                }
+                for (int t = tid; t < 4 * kMaxValence; t += 64) {
+                    const int g = t / kMaxValence, s = t % kMaxValence;
+                    if (s < rn[g]) s_ring[t] = w.nb[(size_t)(4 * kf + g) * kMaxValence + s];
+                }
+            }
+            __syncthreads();
+            if (tid < 4) {
+                const unsigned h = hbit(s_pv[4 * kf + tid]);
+                atomicOr(&s_bits[h >> 5], 1u << (h & 31));
+            }
+            for (int t = tid; t < 4 * kMaxValence; t += 64) {
+                const int g = t / kMaxValence, s = t % kMaxValence;
+                if (s < rn[g]) {
+                    const unsigned h = hbit(s_ring[t]);
+                    atomicOr(&s_bits[h >> 5], 1u << (h & 31));
+                }
+            }
+            __syncthreads();
+            auto maybe = [&](int u) {
+                const unsigned h = hbit(u);
+                return (s_bits[h >> 5] >> (h & 31)) & 1u;
+            };
+            for (int k = kf + 1 + tid; k < cnt; k += 64) {
+                if (s_valid[k] < 0) continue;
+                const int q0 = s_pv[4 * k], q1 = s_pv[4 * k + 1], q2 = s_pv[4 * k + 2], q3 = s_pv[4 * k + 3];
+                const bool own = need_nb && s_valid[k] == 1 &&
+                                 (all || s_nbn[4 * k] < 0 || s_nbn[4 * k + 1] < 0 || s_nbn[4 * k + 2] < 0 ||
+                                  s_nbn[4 * k + 3] < 0);
+                if (!own && !(maybe(q0) || maybe(q1) || maybe(q2) || maybe(q3))) continue;
+                bool hit = own || in_v(q0) || in_v(q1) || in_v(q2) || in_v(q3);
+                if (!hit && need_nb && s_valid[k] == 1) {
+                    for (int g = 0; g < 4 && !hit; g++)
+                        for (int s = 0; s < rn[g] && !hit; s++) {
+                            const int u = s_ring[g * kMaxValence + s];
+                            hit = u == q0 || u == q1 || u == q2 || u == q3;
+                        }
+                }
+                if (hit) s_stale[k] = 1;
+            }
+            pos = kf + 1;
+        } else {
+            if (tid == 0) {
+                for (int c = 0; c < 6; c++) s_P[c] = s_Prej[c];
+                if (ovf < cend) s_err = 2;
+            }
+            pos = cend;
+        }
+        __syncthreads();
+        if (s_err) break;
+    }
+
+    // ---- the committed flips: proposals whose faces no earlier one touched
+    // are disjoint from every earlier flip; the others follow, in order
+    {
+        int fe[kPer], mode[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const int k = 64 * r + tid;
+            fe[r] = e[r];
+            mode[r] = (k < pos && s_valid[k] == 1 && !s_defer[k]) ? (s_accf[k] ? 1 : 2) : 0;
+        }
+        flip_batch(fm, fe, mode);
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int d = 0; d < s_ndef; d++) {
+            const int k = s_dlist[d];
+            if (k >= pos) break;
+            if (s_accf[k]) do_flip(fm, w.e[k]);
+            else do_flip_twice(fm, w.e[k]);
+        }
+    if (tid == 0) {
+        int err = s_err;
+        if (!err && pos < cnt && s_valid[pos] == -1) err = 1;
+        for (int c = 0; c < 6; c++) st->props[c] = s_P[c];
+        st->e0 = s_e0;
+        st->used = used0 + s_rank[pos];
+        st->acc = s_acc;
+        st->err = err;
+        st->pos = pos0 + pos;
+        st->window += 1;
+        if (err || pos0 + pos >= nflips) st->done = 1;
+    }
+}
+
+void launch_serial_init(const Globals* g0, const EParams& p, int nflips, SerialState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sf_init, dim3(1), dim3(64), 0, s, g0, p, nflips, st);
+}
+
+void launch_serial_windows(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips,
+                           const int* proposals, const double* uniforms, int n_uniforms, uint64_t seed, int injected,
+                           int W, SerialState* st, const SerialWindow& w, unsigned long long* vstamp,
+                           unsigned long long* estamp, unsigned long long* fstamp, int n_windows, hipStream_t s)
+{
+    const int need_nb = nl.kind == 0 && nl.exclusion > 1 && p.rep_lc1 > 0;
+    for (int r = 0; r < n_windows; r++) {
+        hipLaunchKernelGGL(k_sf_eval, dim3(W, 2), dim3(64), 0, s, fm, nl, p, nflips, proposals, seed, injected, W,
+                           need_nb, st, w, vstamp, estamp, fstamp);
+        hipLaunchKernelGGL(k_sf_commit, dim3(1), dim3(64), 0, s, fm, p, nflips, uniforms, n_uniforms, seed, injected,
+                           W, need_nb, st, w, vstamp, estamp, fstamp);
+    }
+}
+
 // ============================================ conflict-free parallel sweep ==
 // pflip (flip_parallel_batches, flips.cpp:82-166) lets OpenMP threads lock
 // the edge patch of their proposal (flip_patch, flip_utils.cpp:43-68), skip
@@ -511,7 +1075,7 @@ template <class F>
 __device__ __forceinline__ void for_patch_edges_of(const Topo& t, int e, int w, F&& f)
 {
     int vh = patch_vertex(t, e, w);
-    int h0 = t.v_he[vh];
+    int h0 = t.vhe(vh);
     int h = h0;
     int guard = 0;
     do {

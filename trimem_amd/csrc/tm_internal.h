@@ -89,6 +89,7 @@ struct Tuning {
     int rep_fused = 0;        // 1: the query shares the pass-1 launch (256 threads), 0: its own launch
     int pair_cache = 1;       // 1: replay recorded candidate lists (single meshes, cell list)
     int pflip_window = 256;   // proposals the pflip Metropolis scan tests at once (1..256)
+    int serial_window = 256;  // proposals per window of the serial sweep (1..256), 0: one workgroup
 };
 bool tuning_valid(const Tuning& t);
 int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
@@ -182,6 +183,30 @@ struct FlipMeshView {
 void launch_flip_serial(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips,
                         const int* proposals, const double* uniforms, int n_uniforms, uint64_t seed, int injected,
                         const Globals* g0, int* result /* [accepted, used, error] */, hipStream_t s);
+
+// windowed serial sweep: the serial sweep's decisions and topology from
+// windows of up to kSerialWindowMax proposals evaluated in parallel
+constexpr int kSerialWindowMax = 256;
+struct SerialState {
+    int pos, done, used, acc, err, window;
+    double props[6];
+    double e0;
+};
+struct SerialWindow {
+    int* e;        // [W] proposed edge
+    int* valid;    // [W] 1 flippable, 0 not, -1 index out of range
+    int* pv;       // [4W] vertices whose one-rings the evaluation read (the patch)
+    int* fc;       // [2W] the edge's two faces
+    double* dO;    // [6W] patch properties before the flip
+    double* dN;    // [6W] after the flip
+    int* nbn;      // [4W] one-ring sizes of the patch vertices (-1: above kMaxValence)
+    int* nb;       // [4 kMaxValence W] their one-rings (exclusion level 2 only)
+};
+void launch_serial_init(const Globals* g0, const EParams& p, int nflips, SerialState* st, hipStream_t s);
+void launch_serial_windows(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips,
+                           const int* proposals, const double* uniforms, int n_uniforms, uint64_t seed, int injected,
+                           int W, SerialState* st, const SerialWindow& w, unsigned long long* vstamp,
+                           unsigned long long* estamp, unsigned long long* fstamp, int n_windows, hipStream_t s);
 
 // graph-coloured (claim-based) parallel sweep; state receives props[6], the
 // final energy and the number of accepted flips

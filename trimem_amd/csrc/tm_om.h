@@ -41,6 +41,9 @@ struct Topo {
         return he_face[2 * e] < 0 || he_face[2 * e + 1] < 0;
     }
     __device__ __forceinline__ int cw(int h) const { return he_next[h ^ 1]; }
+    __device__ __forceinline__ int face(int h) const { return he_face[h]; }
+    __device__ __forceinline__ int vhe(int v) const { return v_he[v]; }
+    __device__ __forceinline__ int fhe(int f) const { return f_he[f]; }
 };
 
 __device__ __forceinline__ Topo topo_of(const MeshView& m)
@@ -49,9 +52,11 @@ __device__ __forceinline__ Topo topo_of(const MeshView& m)
 }
 
 // ------------------------------------------------ OpenMesh calc_* helpers --
-__device__ __forceinline__ d3 om_edge_vector(const Topo& t, int h) { return t.P(t.to(h)) - t.P(t.from(h)); }
+template <class TP>
+__device__ __forceinline__ d3 om_edge_vector(const TP& t, int h) { return t.P(t.to(h)) - t.P(t.from(h)); }
 
-__device__ __forceinline__ d3 om_sector_normal(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ d3 om_sector_normal(const TP& t, int h)
 {
     d3 pt = t.P(t.to(h));
     return cross(t.P(t.to(t.next(h))) - pt, t.P(t.from(h)) - pt);
@@ -63,9 +68,10 @@ __device__ __forceinline__ double om_angle(double c, double s)
     return s >= 0 ? acos(c) : -acos(c);
 }
 
-__device__ __forceinline__ d3 om_face_normal(const Topo& t, int f)
+template <class TP>
+__device__ __forceinline__ d3 om_face_normal(const TP& t, int f)
 {
-    int h = t.f_he[f];
+    int h = t.fhe(f);
     d3 p0 = t.P(t.to(h));
     d3 p1 = t.P(t.to(t.next(h)));
     d3 p2 = t.P(t.to(t.next(t.next(h))));
@@ -77,7 +83,8 @@ __device__ __forceinline__ d3 om_face_normal(const Topo& t, int f)
 // OpenMesh is_zero default tolerance for doubles
 __device__ __forceinline__ bool om_is_zero(double a) { return fabs(a) < 100 * 2.220446049250313e-16; }
 
-__device__ __forceinline__ double om_sector_angle(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ double om_sector_angle(const TP& t, int h)
 {
     d3 v0 = om_edge_vector(t, t.next(h));
     d3 v1 = om_edge_vector(t, h ^ 1);
@@ -85,14 +92,15 @@ __device__ __forceinline__ double om_sector_angle(const Topo& t, int h)
     if (om_is_zero(denom)) return 0;
     double c = dot(v0, v1) / denom;
     if (t.he_boundary(h)) {
-        int f = t.he_face[h ^ 1];
+        int f = t.face(h ^ 1);
         d3 fn = f >= 0 ? om_face_normal(t, f) : d3{0, 0, 0};
         return om_angle(c, dot(cross(v0, v1), fn));
     }
     return acos(fmin(1.0, fmax(-1.0, c)));
 }
 
-__device__ __forceinline__ double om_dihedral_angle(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ double om_dihedral_angle(const TP& t, int h)
 {
     if (t.edge_boundary(h >> 1)) return 0;
     d3 n0 = om_sector_normal(t, h);
@@ -104,33 +112,40 @@ __device__ __forceinline__ double om_dihedral_angle(const Topo& t, int h)
 }
 
 // --------------------------------------------------- mesh_util.h:15-185 ----
-__device__ __forceinline__ d3 mu_face_normal(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ d3 mu_face_normal(const TP& t, int h)
 {
     d3 n = om_sector_normal(t, h);
     return n / norm(n);
 }
-__device__ __forceinline__ double mu_edge_length(const Topo& t, int h) { return norm(om_edge_vector(t, h)); }
-__device__ __forceinline__ d3 mu_edge_length_grad_from(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ double mu_edge_length(const TP& t, int h) { return norm(om_edge_vector(t, h)); }
+template <class TP>
+__device__ __forceinline__ d3 mu_edge_length_grad_from(const TP& t, int h)
 {
     d3 e = om_edge_vector(t, h);
     e = e / norm(e);
     return -e;
 }
-__device__ __forceinline__ double mu_face_area(const Topo& t, int h) { return norm(om_sector_normal(t, h)) / 2; }
+template <class TP>
+__device__ __forceinline__ double mu_face_area(const TP& t, int h) { return norm(om_sector_normal(t, h)) / 2; }
 // face_area_grad<N>: [from, to, to(next)] (mesh_util.h:65-92)
-__device__ __forceinline__ d3 mu_face_area_grad(const Topo& t, int h, int which)
+template <class TP>
+__device__ __forceinline__ d3 mu_face_area_grad(const TP& t, int h, int which)
 {
     d3 n = mu_face_normal(t, h);
     n = n / norm(n);
     int g = which == 0 ? t.next(h) : (which == 1 ? t.prev(h) : h);
     return 0.5 * cross(n, om_edge_vector(t, g));
 }
-__device__ __forceinline__ double mu_face_volume(const Topo& t, int h)
+template <class TP>
+__device__ __forceinline__ double mu_face_volume(const TP& t, int h)
 {
     d3 p0 = t.P(t.from(h)), p1 = t.P(t.to(h)), p2 = t.P(t.to(t.next(h)));
     return dot(cross(p1, p2), p0) / 6;
 }
-__device__ __forceinline__ d3 mu_face_volume_grad(const Topo& t, int h, int which)
+template <class TP>
+__device__ __forceinline__ d3 mu_face_volume_grad(const TP& t, int h, int which)
 {
     d3 p0 = t.P(t.from(h)), p1 = t.P(t.to(h)), p2 = t.P(t.to(t.next(h)));
     if (which == 0) return cross(p1, p2) / 6.0;
@@ -139,7 +154,8 @@ __device__ __forceinline__ d3 mu_face_volume_grad(const Topo& t, int h, int whic
 }
 // dihedral_angle_grad<N> for a single bit b in [from, to, opp-in-face,
 // opp-in-twin-face] (mesh_util.h:135-185)
-__device__ __forceinline__ d3 mu_dihedral_angle_grad(const Topo& t, int h, int b)
+template <class TP>
+__device__ __forceinline__ d3 mu_dihedral_angle_grad(const TP& t, int h, int b)
 {
     d3 n0 = mu_face_normal(t, h);
     d3 n1 = mu_face_normal(t, h ^ 1);
@@ -226,11 +242,12 @@ __device__ __forceinline__ double rep_dfac(double l, double lc1, int r)
 
 // Is j inside the exclusion neighbourhood of i (cell_list.h:170-202)?
 // Uses the halfedge circulators, so it sees the current topology.
-__device__ __forceinline__ bool om_excluded(const Topo& t, int i, int j, int excl)
+template <class TP>
+__device__ __forceinline__ bool om_excluded(const TP& t, int i, int j, int excl)
 {
     if (i == j) return true;
     if (excl <= 0) return false;
-    int h0 = t.v_he[i];
+    int h0 = t.vhe(i);
     if (h0 < 0) return false;
     int h = h0;
     int guard = 0;
@@ -238,7 +255,7 @@ __device__ __forceinline__ bool om_excluded(const Topo& t, int i, int j, int exc
         int vi = t.to(h);
         if (vi == j) return true;
         if (excl > 1) {
-            int g0 = t.v_he[vi];
+            int g0 = t.vhe(vi);
             int g = g0;
             int guard2 = 0;
             do {
@@ -313,11 +330,12 @@ struct VP {
     double v[6];
 };
 
-__device__ inline VP om_vertex_properties(const Topo& t, const NListView& nl, const EParams& p, int ve)
+template <class TP>
+__device__ inline VP om_vertex_properties(const TP& t, const NListView& nl, const EParams& p, int ve)
 {
     VP r;
     double area = 0, vol = 0, curv = 0, teth = 0;
-    int h0 = t.v_he[ve];
+    int h0 = t.vhe(ve);
     if (h0 >= 0) {
         int h = h0;
         int guard = 0;
@@ -357,12 +375,13 @@ __device__ inline VP om_vertex_properties(const Topo& t, const NListView& nl, co
 // Gradient of trimem's energy at vertex ve gathered as in
 // mesh_properties.cpp:51-148 and combined as in kernel.h:124-137.
 // q[v] = curvature/area of vertex v.
-__device__ inline d3 om_vertex_gradient(const Topo& t, const NListView& nl, const EParams& p, const Globals& g,
+template <class TP>
+__device__ inline d3 om_vertex_gradient(const TP& t, const NListView& nl, const EParams& p, const Globals& g,
                                         const double* __restrict__ q, int ve)
 {
     d3 dA{0, 0, 0}, dV{0, 0, 0}, dC{0, 0, 0}, dB{0, 0, 0}, dT{0, 0, 0}, dR{0, 0, 0};
     double c0 = q[ve];
-    int h0 = t.v_he[ve];
+    int h0 = t.vhe(ve);
     if (h0 >= 0) {
         int he = h0;
         int guard = 0;
