@@ -1272,7 +1272,7 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
         HIPCHECK(hipMemsetAsync(em->pf_owner.p, 0, sizeof(unsigned long long) * ne, g_rt.stream));
     }
     em->pf_status.alloc(batch);
-    em->pf_dprops.alloc((size_t)6 * batch);
+    em->pf_dprops.alloc((size_t)12 * batch);    // -old, then new
     em->pf_state.alloc(8);
     if (nflips > 0)
         launch_flip_parallel(mesh->flip_view(), em->nlist->view(), eparams_of(*em), nflips, batch,

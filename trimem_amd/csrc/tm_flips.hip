@@ -1057,12 +1057,13 @@ void launch_serial_windows(const FlipMeshView& fm, const NListView& nl, const EP
 //           the owner array never needs clearing);
 //   eval    a proposal whose key survived on its whole patch is a winner;
 //           winners have pairwise disjoint patches.  Winners that pass
-//           is_flip_ok evaluate the patch properties before the flip;
-//   apply   winners flip;
-//   eval2   winners evaluate the patch after the flip: delta = new - old;
-//   scan    one lane walks the winners in proposal order and applies the
-//           FP64 Metropolis test to the running global properties;
-//   revert  rejected winners flip back.
+//           is_flip_ok evaluate the patch properties before the flip and,
+//           on FlippedTopo, after it: delta = new - old.  The evaluation
+//           after the flip sees the proposal's own flip only (the
+//           reference's threads see whichever other flips have happened);
+//   scan    the FP64 Metropolis test in proposal order on the running
+//           global properties;
+//   apply   accepted winners flip.
 // Proposals that lose the claim are skipped, like the reference's failed
 // locks.  Every phase is its own kernel, so no phase reads topology that
 // another thread is rewriting; the result depends only on the seed.
@@ -1110,21 +1111,23 @@ __global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t se
 }
 
 // status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected.
-// One workgroup per proposal: lanes 0-3 check that the proposal's key
-// survived on their part of the patch while lane 4 tests is_flip_ok; a
-// winner's workgroup then evaluates the patch.
+// One workgroup per proposal and side: lanes 0-3 check that the proposal's
+// key survived on their part of the patch while lane 4 tests is_flip_ok; a
+// winner's workgroups then evaluate the patch before the flip (side 0,
+// dold = -old) and after it on FlippedTopo (side 1, dnew = new).
 __global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
                                                    int round, int B, int nprop,
                                                    const unsigned long long* __restrict__ owner,
-                                                   int* __restrict__ status, double* __restrict__ dprops)
+                                                   int* __restrict__ status, double* __restrict__ dold,
+                                                   double* __restrict__ dnew)
 {
     __shared__ CoopShared S;
     __shared__ int s_go;
-    const int i = blockIdx.x;
+    const int i = blockIdx.x, side = blockIdx.y;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
     const int e = pflip_edge(seed, round, B, i, fm.nh / 2);
     if (threadIdx.x == 0) {
-        status[i] = 0;
+        if (side == 0) status[i] = 0;
         s_go = 1;
     }
     if (t.edge_boundary(e)) return;
@@ -1139,10 +1142,17 @@ __global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl
     }
     __syncthreads();
     if (!s_go) return;
-    coop_patch(t, nl, p, e, S);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) dprops[6 * i + k] = -S.sum[k];
-        status[i] = 1;
+    if (side == 0) {
+        coop_patch(t, nl, p, e, S);
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 6; k++) dold[6 * i + k] = -S.sum[k];
+            status[i] = 1;
+        }
+    } else {
+        const FlippedTopo ft(t, e);
+        coop_patch(ft, nl, p, e, S);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 6; k++) dnew[6 * i + k] = S.sum[k];
     }
 }
 
@@ -1152,19 +1162,6 @@ __global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, 
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nprop || status[i] != want) return;
     do_flip(fm, pflip_edge(seed, round, B, i, fm.nh / 2));
-}
-
-__global__ void __launch_bounds__(64) k_pflip_eval2(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
-                                                    int round, int B, int nprop, const int* __restrict__ status,
-                                                    double* __restrict__ dprops)
-{
-    __shared__ CoopShared S;
-    const int i = blockIdx.x;
-    if (status[i] != 1) return;
-    const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
-    coop_patch(t, nl, p, pflip_edge(seed, round, B, i, fm.nh / 2), S);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 6; k++) dprops[6 * i + k] += S.sum[k];
 }
 
 // state: props[6], e0, accepted count (as double).  The Metropolis test
@@ -1182,7 +1179,8 @@ constexpr int kScanChunk = 1024;
 
 __global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop,
                                                              int window, int* __restrict__ status,
-                                                             const double* __restrict__ dprops,
+                                                             const double* __restrict__ dold,
+                                                             const double* __restrict__ dnew,
                                                              double* __restrict__ state)
 {
     __shared__ int s_st[kScanChunk];
@@ -1201,7 +1199,11 @@ __global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t
                 s_u[k] = rng_uniform(seed ^ 0x5bd1e9955bd1e995ull,
                                      (uint64_t)round * (uint64_t)B + (uint64_t)(base + k));
         }
-        for (int k = tid; k < 6 * cnt; k += kScanThreads) s_dp[k] = dprops[6 * (size_t)base + k];
+        for (int k = tid; k < 6 * cnt; k += kScanThreads) {
+            double d = dold[6 * (size_t)base + k];
+            d += dnew[6 * (size_t)base + k];
+            s_dp[k] = d;
+        }
         if (tid == 0) s_first[0] = s_first[1] = cnt;
         __syncthreads();
         // s_first[it & 1] collects iteration it's first acceptance; thread 0
@@ -1256,14 +1258,12 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
         int nprop = std::min(batch, nflips - r * batch);
         dim3 grid((nprop + 127) / 128), block(128);
         hipLaunchKernelGGL(k_pflip_claim, dim3((nprop + 15) / 16), dim3(64), 0, s, fm, seed, r, batch, nprop, owner);
-        hipLaunchKernelGGL(k_pflip_eval, dim3(nprop), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, owner, status,
-                           dprops);
-        hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 1);
-        hipLaunchKernelGGL(k_pflip_eval2, dim3(nprop), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, status,
-                           dprops);
+        hipLaunchKernelGGL(k_pflip_eval, dim3(nprop, 2), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, owner,
+                           status, dprops, dprops + 6 * (size_t)batch);
         hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop,
-                           std::min(std::max(window, 1), kScanThreads), status, dprops, state);
-        hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 3);
+                           std::min(std::max(window, 1), kScanThreads), status, dprops, dprops + 6 * (size_t)batch,
+                           state);
+        hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 2);
     }
 }
 
