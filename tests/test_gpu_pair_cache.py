@@ -85,7 +85,7 @@ def test_cache_is_bit_identical(core, excl, precision):
         x = mesh.points()
         kind = k % 4
         if kind in (0, 1):
-            # small moves: replay (max |dx| ~ 0.05 lbar < 0.45 * 0.2 * 1.5 lbar);
+            # small moves: replay (max |dx| ~ 0.05 lbar < 0.25 * 0.2 * 1.5 lbar);
             # a few percent of the vertices cross a cell boundary
             x[:] = x + rng.normal(0, 0.01 * lbar, x.shape)
         elif kind == 2:

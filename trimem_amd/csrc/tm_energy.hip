@@ -971,7 +971,10 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         const int n = meta_valence(m.meta[u]);
         RepTerms tr;
         auto add = [&](int, d3 d) { tr.add(d, p); };
-        if (mode == 2) record_neighbours<kBatch>(m, A.nl, pc, u, n, xu, p.rep_lc1, lane, add);
+        // a replay launch records the lists of the vertices that left
+        // their recorded cell (in the cell they are in now), so each pays
+        // the full query once per cell it enters
+        if (mode != 0) record_neighbours<kBatch>(m, A.nl, pc, u, n, xu, p.rep_lc1, lane, add);
         else group_neighbours<kBatch>(m, A.nl, u, n, xu, p.rep_lc1, lane, add);
         tr.group_reduce();
         if (lane == 0) {

@@ -21,7 +21,12 @@ constexpr int kPairLanes = 8;      // lanes per vertex of the query
 constexpr int kPairCap = 4;        // recorded candidates per lane (more: that vertex takes the full query)
 constexpr int kDispBlocks = 128;   // blocks of k_disp, one partial each
 constexpr double kPairSkin = 0.2;  // skin = 0.2 lc1
-constexpr double kPairLimit = 0.45;    // replay while every vertex has moved < 0.45 skin (< skin / 2)
+// Replay while every vertex has moved less than 0.25 skin from the snapshot
+// of the last recording of all lists.  A vertex that left the cell its list
+// was recorded in records it again there (the snapshot stays), so two lists
+// can be recorded at different times: a pair's distance then changed by at
+// most 4 x 0.25 skin since either list was recorded.
+constexpr double kPairLimit = 0.25;
 struct PairCache {
     double* x_snap;         // [3N] positions at the recording; null: no cache
     int* cell;              // [N] cell id at the recording, -1: replay not allowed
