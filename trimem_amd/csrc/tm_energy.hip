@@ -884,6 +884,140 @@ __global__ void __launch_bounds__(kThreads) k_rep(PassArgs A)
 // Without cache buffers (a replica with a Verlet list in a cached group
 // launch) every ring vertex takes phase 2 without recording (mode 0).
 constexpr int kVertsPerBlock = kCachedVerts;
+
+// record_neighbours by one 64-lane wave instead of an 8-lane group, with
+// the same result bits.  The candidates are laid out as group_neighbours
+// walks them: runs (one range of the vertex's stored stencil list, with the
+// exclusion masks when it is in its build-time cell; the stored list of the
+// occupied cell it moved to; or the nine stencil rows of cell_verts), lane l
+// of the group visiting indices l, l + 8, ... of each run in turn.  Lane L of
+// the wave tests candidate L, L + 64, ... of the concatenated runs and
+// computes its terms; then, in candidate order, the lanes whose candidate is
+// within lc1 + skin pass it (lane shuffles) to lane (index in its run) % 8,
+// which records and adds it as RepTerms::add would.  Lanes 0-7 end with the
+// group's lane state; the caller reduces them with group_sum.
+__device__ __forceinline__ void wide_record(const MeshView& m, const NListView& nl, const PairCache& pc,
+                                            const EParams& p, int u, int n, d3 pu, RepTerms& tr)
+{
+    const int L = threadIdx.x & 63;
+    int rs[9], rl[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) rs[k] = rl[k] = 0;
+    const int* ids = nl.cand;
+    bool masked = false;
+    unsigned long long m0 = 0ull, m1 = 0ull;
+    {
+        int2 vr = make_int2(0, 0);
+        int vc = -1;
+        if (nl.vcand) {
+            vr = nl.vcand[u];
+            vc = nl.vert_cell[u];
+        }
+        int c[3];
+        cell_coords(nl, pu, c);
+        const bool inside = c[0] >= 0 && c[0] < nl.shape[0] && c[1] >= 0 && c[1] < nl.shape[1] && c[2] >= 0 &&
+                            c[2] < nl.shape[2];
+        const int flat = c[0] * nl.strides[0] + c[1] * nl.strides[1] + c[2] * nl.strides[2];
+        bool done = false;
+        if (inside && flat == vc) {
+            rs[0] = vr.x;
+            rl[0] = vr.y - vr.x;
+            if (nl.excl_mask) {
+                masked = true;
+                m0 = nl.excl_mask[2 * u];
+                m1 = nl.excl_mask[2 * u + 1];
+            }
+            done = true;
+        } else if (inside && nl.cand_start) {
+            const int s0 = nl.cell_start[flat];
+            if (nl.cell_start[flat + 1] > s0) {
+                rs[0] = nl.cand_start[s0];
+                rl[0] = nl.cand_start[s0 + 1] - rs[0];
+                done = true;
+            }
+        }
+        if (!done) {
+            ids = nl.cell_verts;
+            const int i0 = max(c[0] - 1, 0), i1 = min(c[0] + 1, nl.shape[0] - 1);
+#pragma unroll
+            for (int r = 0; r < 9; r++) {
+                const int kk = c[2] + r / 3 - 1, jj = c[1] + r % 3 - 1;
+                if (i0 <= i1 && kk >= 0 && kk < nl.shape[2] && jj >= 0 && jj < nl.shape[1]) {
+                    const int base = jj * nl.strides[1] + kk * nl.strides[2];
+                    rs[r] = nl.cell_start[base + i0];
+                    rl[r] = nl.cell_start[base + i1 + 1] - rs[r];
+                }
+            }
+        }
+    }
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) total += rl[k];
+    const RingRegs R = load_ring(m, u, n);
+    const double dsk = p.rep_lc1 + pc.skin, dsk2 = dsk * dsk * (1.0 + 1e-15);
+    const double dmax = p.rep_lc1, dmax2 = dmax * dmax * (1.0 + 1e-15);
+    int* ent = pc.ent + ((size_t)u * kGroup + (L & 7)) * kPairCap;
+    int rec = 0;
+    for (int t0 = 0; t0 < total; t0 += 64) {
+        const int t = t0 + L;
+        int j = -1, flag = 0, owner = 0;
+        double val = 0.0;
+        d3 gv{0.0, 0.0, 0.0};
+        if (t < total) {
+            // run and index of candidate t
+            int i = t, q = -1;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                if (q < 0 && i < rl[k]) q = rs[k] + i;
+                if (q < 0) i -= rl[k];
+            }
+            owner = i & 7;
+            // the masks cover the first 128 of the stored list: 1 excluded,
+            // 0 not (no ring test needed), -1 beyond (ring test)
+            const int ex = masked ? (i < 64 ? (int)((m0 >> i) & 1ull) : (i < 128 ? (int)((m1 >> (i - 64)) & 1ull) : -1))
+                                  : -1;
+            if (ex != 1) {
+                j = ids[q];
+                const d3 d = pu - load_p(m.x, j);
+                const double sq = sqnorm(d);
+                if (sq <= dsk2 && sqrt(sq) < dsk && (ex == 0 || !excluded_fast(m, R, u, j, nl.exclusion))) {
+                    flag = 1;
+                    if (sq <= dmax2 && sqrt(sq) < dmax) {
+                        const double len = norm(d);
+                        val = rep_value(len, p.rep_lc1, p.rep_r);
+                        gv = (p.kappa_r * 2.0 * rep_dfac(len, p.rep_lc1, p.rep_r)) * d;
+                        flag = 3;
+                    }
+                }
+            }
+        }
+        unsigned long long bits = __ballot(flag != 0);
+        while (bits) {
+            const int src = __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
+            const int fo = __shfl(owner, src), ff = __shfl(flag, src), fj = __shfl(j, src);
+            const double fv = __shfl(val, src);
+            const double gx = __shfl(gv.x, src), gy = __shfl(gv.y, src), gz = __shfl(gv.z, src);
+            if (L == fo) {
+                if (rec < kPairCap) ent[rec] = fj;
+                rec++;
+                if (ff & 2) {
+                    tr.rep += fv;
+                    tr.g += d3{gx, gy, gz};
+                    tr.cnt++;
+                }
+            }
+        }
+    }
+    const int over = group_sum(rec > kPairCap ? 1 : 0);
+    const int word = group_sum(min(rec, kPairCap) << (4 * (L & 7)));
+    const int cell = query_cell(nl, pu);
+    if (L == 0) {
+        pc.cnt[u] = (unsigned)word;
+        pc.cell[u] = over == 0 ? cell : -1;
+    }
+}
+
 template <int kBatch>
 __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
 {
@@ -963,10 +1097,40 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         }
     }
     __syncthreads();
-    const int n_full = s_n;
+    int n_full = s_n;
+    const int* list = s_list;
+    __shared__ int s_rest[kVertsPerBlock];
+    __shared__ int s_nrest;
+    if (mode == 1) {
+        // a replay launch leaves few vertices per block to the full query,
+        // so its latency counts: each takes a whole wave (wide_record)
+        if (threadIdx.x == 0) s_nrest = 0;
+        __syncthreads();
+        const int wv = threadIdx.x / 64;
+        for (int i = wv; i < n_full; i += 256 / 64) {
+            const int u = s_list[i];
+            const d3 xu = load_p(m.x, u);
+            if (A.nl.kind == 0) {
+                RepTerms tr;
+                wide_record(m, A.nl, pc, p, u, meta_valence(m.meta[u]), xu, tr);
+                tr.group_reduce();
+                if ((threadIdx.x & 63) == 0) {
+                    A.rep_v[u] = tr.rep;
+                    A.pair_cnt[u] = tr.cnt;
+                    s_val[u - base] = tr.rep;
+                    store_rep_g(A, u, tr);
+                }
+            } else if ((threadIdx.x & 63) == 0) {
+                s_rest[atomicAdd(&s_nrest, 1)] = u;
+            }
+        }
+        __syncthreads();
+        n_full = s_nrest;
+        list = s_rest;
+    }
     const int g = threadIdx.x / kGroup, lane = threadIdx.x % kGroup;
     for (int i = g; i < n_full; i += 256 / kGroup) {
-        const int u = s_list[i];
+        const int u = list[i];
         const d3 xu = load_p(m.x, u);
         const int n = meta_valence(m.meta[u]);
         RepTerms tr;
