@@ -570,15 +570,16 @@ def main():
                          "rocprofv3 output folders and exit")
     args = ap.parse_args()
     if args.summarize:
+        only = " --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0 --flips 0 --combined 0"
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
-               " -o run -- python3 bench.py --steps 200 --warmup 10 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0",
+               " -o run -- python3 bench.py --steps 200 --warmup 10" + only,
                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + args.summarize[1] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0",
+               " -o run -- python3 bench.py --steps 20 --warmup 2" + only,
                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + args.summarize[2] +
-               " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0"]
+               " -o run -- python3 bench.py --steps 20 --warmup 2" + only]
         if len(args.summarize) > 3:
             cmd.append("rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d " +
-                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0")
+                       args.summarize[3] + " -o run -- python3 bench.py --steps 20 --warmup 2" + only)
         cmd.append("python3 bench.py --summarize " + " ".join(args.summarize))
         summarize(args.summarize[0], args.summarize[1], args.summarize[2], PROFILE_JSON,
                   os.path.join(ROOT, "profiles", "energy_passes_L7_mixed.md"), cmd,

@@ -1037,7 +1037,12 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
     __shared__ int s_list[kVertsPerBlock];
     __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (pr[5] of pass 1)
     __shared__ int s_n;
-    if (threadIdx.x == 0) s_n = 0;
+    __shared__ int s_rest[kVertsPerBlock];
+    __shared__ int s_nrest;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_nrest = 0;
+    }
     __syncthreads();
     if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
     if (mine) {
@@ -1099,13 +1104,9 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
     __syncthreads();
     int n_full = s_n;
     const int* list = s_list;
-    __shared__ int s_rest[kVertsPerBlock];
-    __shared__ int s_nrest;
-    if (mode == 1) {
+    if (mode == 1 && n_full > 0) {
         // a replay launch leaves few vertices per block to the full query,
         // so its latency counts: each takes a whole wave (wide_record)
-        if (threadIdx.x == 0) s_nrest = 0;
-        __syncthreads();
         const int wv = threadIdx.x / 64;
         for (int i = wv; i < n_full; i += 256 / 64) {
             const int u = s_list[i];
