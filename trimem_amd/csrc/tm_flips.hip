@@ -1,8 +1,10 @@
-// tm_flips.hip -- ring-table rebuild and the serial edge-flip sweep.
+// tm_flips.hip -- ring-table rebuild and the edge-flip sweeps: the serial
+// sweep (one workgroup, k_flip_serial, or windows of proposals evaluated at
+// once and committed in order, k_sf_*) and the parallel sweep (k_pflip_*).
 //
 // flip_serial (src/flips.cpp:22-80) is a Metropolis loop in which every
-// proposal depends on the topology left by the previous one, so it runs as
-// one workgroup walking the proposal stream in order.  For each proposal:
+// proposal depends on the topology left by the previous one.  The
+// one-workgroup kernel walks the proposal stream in order.  For each proposal:
 //   * lane 0 tests OpenMesh is_flip_ok and the boundary condition;
 //   * the workgroup evaluates vertex_properties of the four patch vertices
 //     (flip_utils.cpp:19-41), 16 lanes per vertex (coop_patch), including
