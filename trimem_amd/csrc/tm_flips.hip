@@ -616,6 +616,9 @@ __device__ __forceinline__ unsigned long long sf_key(int window, int W, int k)
     return ((unsigned long long)(unsigned)(window + 1) << 32) | (unsigned)(W - k);
 }
 
+// bitmap position (12 bits) of a vertex in the commit's staleness filter
+__device__ __forceinline__ unsigned sf_hash(int u) { return ((unsigned)u * 2654435761u) >> 20; }
+
 // window position recorded in a stamp, or W if the stamp is older
 __device__ __forceinline__ int sf_pos(unsigned long long key, unsigned wid, int W)
 {
@@ -788,7 +791,10 @@ __global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, in
     __shared__ int s_nbn[4 * kSerialWindowMax];
     __shared__ int s_ring[4 * kMaxValence];     // one-rings of an accepted proposal's patch
     __shared__ int s_dlist[kSerialWindowMax];   // positions of the deferred flips, ascending
-    __shared__ unsigned s_bits[32];             // hash bitmap of s_ring and the accepted patch
+    __shared__ unsigned s_bits[128];            // hash bitmap of s_ring and the accepted patch
+    // per proposal: the bitmap positions of its patch vertices (12 bits each)
+    // and, in bit 63, whether one of its one-rings was too long to record
+    __shared__ unsigned long long s_ph[kSerialWindowMax];
     // by rank among the valid proposals: uniforms, the rounding path's
     // states and the patch properties before and after the flip
     __shared__ double s_u[kSerialWindowMax];
@@ -829,15 +835,35 @@ __global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, in
             stale |= sf_pos(estamp[e[r]], wid, W) < k;
             for (int g = 0; g < 4; g++) s_pv[4 * k + g] = pv[r][g];
         }
+        unsigned long long ph = 0ull;
+        if (v[r] >= 0)
+            for (int g = 0; g < 4; g++) ph |= (unsigned long long)sf_hash(pv[r][g]) << (12 * g);
         if (v[r] == 1) {
             defer = sf_pos(fstamp[fc[r][0]], wid, W) < k || sf_pos(fstamp[fc[r][1]], wid, W) < k;
             if (need_nb)
-                for (int g = 0; g < 4; g++) s_nbn[4 * k + g] = w.nbn[4 * k + g];
+                for (int g = 0; g < 4; g++) {
+                    const int nn = w.nbn[4 * k + g];
+                    s_nbn[4 * k + g] = nn;
+                    if (nn < 0) ph |= 1ull << 63;
+                }
         }
+        s_ph[k] = ph;
         s_valid[k] = v[r];
         s_stale[k] = stale;
         s_defer[k] = defer;
         s_accf[k] = 0;
+    }
+    // the patch properties of the valid proposals, all rows in flight at once
+    // (one workgroup: registers are not the limit)
+    double po[kPer][6], pn[kPer][6];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int k = 64 * r + tid;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            po[r][c] = v[r] == 1 ? w.dO[6 * k + c] : 0.0;
+            pn[r][c] = v[r] == 1 ? w.dN[6 * k + c] : 0.0;
+        }
     }
     // ranks among the valid proposals (their uniform index is used0 + rank)
     // and the list of deferred flips
@@ -857,8 +883,8 @@ __global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, in
             s_vk[m] = k;
             s_u[m] = injected ? (u < n_uniforms ? uniforms[u] : 2.0) : rng_uniform(seed, (uint64_t)u);
             for (int c = 0; c < 6; c++) {
-                s_o[6 * m + c] = w.dO[6 * k + c];
-                s_n[6 * m + c] = w.dN[6 * k + c];
+                s_o[6 * m + c] = po[r][c];
+                s_n[6 * m + c] = pn[r][c];
             }
         }
         base += __popcll(b);
@@ -935,10 +961,9 @@ __global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, in
             // patch's one-rings), found in the accepted patch's one-rings
             const int f0 = s_pv[4 * kf], f1 = s_pv[4 * kf + 1], f2 = s_pv[4 * kf + 2], f3 = s_pv[4 * kf + 3];
             auto in_v = [&](int u) { return u == f0 || u == f1 || u == f2 || u == f3; };
-            auto hbit = [](int u) { return ((unsigned)u * 2654435761u) >> 22; };    // 10 bits
             int rn[4] = {0, 0, 0, 0};
             bool all = false;
-            if (tid < 32) s_bits[tid] = 0u;
+            for (int t = tid; t < 128; t += 64) s_bits[t] = 0u;
             if (need_nb) {
                 for (int g = 0; g < 4; g++) {
                     rn[g] = s_nbn[4 * kf + g];
@@ -951,28 +976,27 @@ __global__ void __launch_bounds__(64) k_sf_commit(FlipMeshView fm, EParams p, in
             }
             __syncthreads();
             if (tid < 4) {
-                const unsigned h = hbit(s_pv[4 * kf + tid]);
+                const unsigned h = sf_hash(s_pv[4 * kf + tid]);
                 atomicOr(&s_bits[h >> 5], 1u << (h & 31));
             }
             for (int t = tid; t < 4 * kMaxValence; t += 64) {
                 const int g = t / kMaxValence, s = t % kMaxValence;
                 if (s < rn[g]) {
-                    const unsigned h = hbit(s_ring[t]);
+                    const unsigned h = sf_hash(s_ring[t]);
                     atomicOr(&s_bits[h >> 5], 1u << (h & 31));
                 }
             }
             __syncthreads();
-            auto maybe = [&](int u) {
-                const unsigned h = hbit(u);
+            auto maybe = [&](unsigned long long ph, int g) {
+                const unsigned h = (unsigned)(ph >> (12 * g)) & 0xfffu;
                 return (s_bits[h >> 5] >> (h & 31)) & 1u;
             };
             for (int k = kf + 1 + tid; k < cnt; k += 64) {
                 if (s_valid[k] < 0) continue;
+                const unsigned long long ph = s_ph[k];
+                const bool own = need_nb && s_valid[k] == 1 && (all || (ph >> 63));
+                if (!own && !(maybe(ph, 0) | maybe(ph, 1) | maybe(ph, 2) | maybe(ph, 3))) continue;
                 const int q0 = s_pv[4 * k], q1 = s_pv[4 * k + 1], q2 = s_pv[4 * k + 2], q3 = s_pv[4 * k + 3];
-                const bool own = need_nb && s_valid[k] == 1 &&
-                                 (all || s_nbn[4 * k] < 0 || s_nbn[4 * k + 1] < 0 || s_nbn[4 * k + 2] < 0 ||
-                                  s_nbn[4 * k + 3] < 0);
-                if (!own && !(maybe(q0) || maybe(q1) || maybe(q2) || maybe(q3))) continue;
                 bool hit = own || in_v(q0) || in_v(q1) || in_v(q2) || in_v(q3);
                 if (!hit && need_nb && s_valid[k] == 1) {
                     for (int g = 0; g < 4 && !hit; g++)
