@@ -69,10 +69,35 @@ __global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n
     __syncthreads();
     if (!s_last) return;
     __threadfence();
+    // the last block combines the blocks' partials, all of them in flight at
+    // once (min and max are exact in any order)
+    for (int k = 0; k < 3; k++) {
+        mn[k] = DBL_MAX;
+        mx[k] = -DBL_MAX;
+    }
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(mn[k], part[b * 6 + k]);
+            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+        }
+    }
+    __syncthreads();    // s was last read before the counter barrier; reuse it
+    if (lane == 0) {
+        for (int k = 0; k < 3; k++) {
+            s[k][wave] = mn[k];
+            s[3 + k][wave] = mx[k];
+        }
+    }
+    __syncthreads();
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
-        double r = part[k];
-        for (int b = 1; b < (int)gridDim.x; b++) r = k < 3 ? fmin(r, part[b * 6 + k]) : fmax(r, part[b * 6 + k]);
+        double r = s[k][0];
+        for (int w = 1; w < 4; w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
         box[k] = r;
     }
     if (threadIdx.x == 0) *counter = 0u;
