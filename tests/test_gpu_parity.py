@@ -321,6 +321,35 @@ def test_serial_flip_windows_match_one_workgroup(core, rep, excl):
         assert runs[w][2] == runs[0][2], w
 
 
+@pytest.mark.parametrize("w", [0, 7, 256])
+def test_flip_serial_open_tube(core, w):
+    """Serial sweeps over an injected stream on an open tube (boundary edges
+    are never flipped, and a boundary proposal's window entry covers only
+    its endpoints) give the oracle's accepted count, consumed uniforms and
+    faces, for the one-workgroup sweep and windows of 7 and 256; cell list
+    at exclusion level 2."""
+    v, f = M.tube(length=1.0, radius=0.5, segments=24)
+    lbar = M.mean_edge_length(v, f)
+    v = v + np.random.default_rng(3).normal(0, 0.03 * lbar, v.shape)
+    p = O.Params(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0, kappa_r=1.0,
+                 bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar,
+                 rep_exclusion=2)
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, p)
+    gm = core.TriMesh(v, f)
+    ge = with_window(core, gm, p, w)
+    ne = gm.n_edges()
+    rng = np.random.default_rng(4)
+    prop = rng.integers(0, ne, ne).astype(np.int32)
+    uni = rng.random(ne)
+    acc_o, used_o = oe.flip_serial_injected(om, prop, uni)
+    acc_g, used_g = core.flip_injected(gm, ge, prop, uni)
+    print(w, acc_o, used_o, flush=True)
+    assert used_o < ne and acc_o > 0          # boundary proposals consume no uniform
+    assert (acc_g, used_g) == (acc_o, used_o)
+    assert np.array_equal(gm.fv_indices(), om.fv_indices())
+
+
 def test_serial_flip_errors_match_one_workgroup(core):
     """A proposal index out of range stops the sweep at that proposal, and an
     exhausted uniform stream after the proposal that found it empty (which
