@@ -350,6 +350,29 @@ def test_flip_serial_open_tube(core, w):
     assert np.array_equal(gm.fv_indices(), om.fv_indices())
 
 
+def test_pflip_open_tube_consistent_with_oracle(core):
+    """Parallel sweeps on an open tube (boundary edges are never claimed)
+    leave a device topology whose properties, energy and gradient match an
+    oracle mesh rebuilt from its faces (area/volume/curvature penalties off,
+    so the two managers' reference values do not enter)."""
+    v, f = M.tube(length=1.0, radius=0.5, segments=24)
+    lbar = M.mean_edge_length(v, f)
+    v = v + np.random.default_rng(5).normal(0, 0.03 * lbar, v.shape)
+    p = O.Params(kappa_b=1.0, kappa_a=0.0, kappa_v=0.0, kappa_c=0.0, kappa_t=1.0, kappa_r=1.0,
+                 bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar,
+                 rep_exclusion=2)
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    accs = [core.pflip(gm, ge, 0.5, seed=300 + k) for k in range(4)]
+    fv = gm.fv_indices()
+    assert sum(accs) > 0 and not np.array_equal(fv, f)
+    om = O.Mesh(v, fv)
+    oe = O.EnergyManager(om, p)
+    assert rel(props_vec(ge.properties(gm)), oe.properties(om)) < 1e-10
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) < 1e-10
+
+
 def test_serial_flip_errors_match_one_workgroup(core):
     """A proposal index out of range stops the sweep at that proposal, and an
     exhausted uniform stream after the proposal that found it empty (which
