@@ -481,8 +481,11 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     L.fill.alloc(nc);
     L.vert_cell.alloc(n);
     L.cell_verts.alloc(n);
-    HIPCHECK(hipMemsetAsync(L.counts.p, 0, sizeof(int) * (nc + 1), g_rt.stream));
-    HIPCHECK(hipMemsetAsync(L.fill.p, 0, sizeof(int) * nc, g_rt.stream));
+    const bool cells = L.kind == 0 && n > 0;    // the per-cell candidate lists
+    if (cells) L.cand_count.alloc(n + 1);
+    launch_fill_ints(L.counts.p, nc + 1, 0, L.fill.p, nc, 0, cells ? L.cand_count.p + n : nullptr, cells ? 1 : 0, 0,
+                     g_rt.stream);
+    check_launch();
     launch_cell_ids(M.x.p, n, v, L.vert_cell.p, L.counts.p, g_rt.stream);
     check_launch();
     scan(L.counts.p, L.cell_start.p, nc + 1, L.scan_tmp);
@@ -496,9 +499,7 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     v.perm = M.d_perm.p;
     if (L.kind == 0 && n > 0) {
         if ((long long)27 * n > INT_MAX) throw TmError(TM_E_UNSUPPORTED, "mesh too large for the candidate table");
-        L.cand_count.alloc(n + 1);
         L.cand_start.alloc(n + 1);
-        HIPCHECK(hipMemsetAsync(L.cand_count.p + n, 0, sizeof(int), g_rt.stream));
         launch_cand_count(v, n, L.cand_count.p, g_rt.stream);
         check_launch();
         scan(L.cand_count.p, L.cand_start.p, n + 1, L.scan_tmp);
@@ -541,8 +542,10 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         L.pc_cnt.alloc(n);
         L.pc_ent.alloc((size_t)kPairLanes * kPairCap * n);
         L.pc_disp.alloc(kDispBlocks);
-        HIPCHECK(hipMemsetAsync(L.pc_cell.p, 0xFF, sizeof(int) * n, g_rt.stream));    // -1: no replay
-        HIPCHECK(hipMemsetAsync(L.pc_disp.p, 0x7F, sizeof(double) * kDispBlocks, g_rt.stream));    // ~1e306
+        // cell -1: no replay; displacement bytes 0x7F (~1e306): a recording first
+        launch_fill_ints(L.pc_cell.p, n, -1, reinterpret_cast<int*>(L.pc_disp.p), 2 * kDispBlocks, 0x7F7F7F7F,
+                         nullptr, 0, 0, g_rt.stream);
+        check_launch();
         L.pc_ok = false;
     }
     // no wait here: everything that reads the list is queued behind it on the

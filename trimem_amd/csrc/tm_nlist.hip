@@ -105,6 +105,23 @@ __global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n
 
 int box_scratch_doubles() { return kBoxBlocks * 6 + 8; }
 
+// Up to three int ranges set to a value each, in one launch (the list build
+// issues these fills back to back after its one host round trip).
+__global__ void k_fill_ints(int* a, int na, int va, int* b, int nb, int vb, int* c, int nc, int vc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = va;
+    if (i < nb) b[i] = vb;
+    if (i < nc) c[i] = vc;
+}
+
+void launch_fill_ints(int* a, int na, int va, int* b, int nb, int vb, int* c, int nc, int vc, hipStream_t s)
+{
+    const int n = std::max(na, std::max(nb, nc));
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_ints, dim3((n + 255) / 256), dim3(256), 0, s, a, na, va, b, nb, vb, c, nc, vc);
+}
+
 void launch_box(const double* x, int n, double* scratch, unsigned* counter, double* box6, hipStream_t s)
 {
     hipLaunchKernelGGL(k_box, dim3(kBoxBlocks), dim3(256), 0, s, x, n, scratch, counter, box6);
