@@ -9,6 +9,12 @@ with exclusion level 2 and rc = rlist = 1.5 l.  One step = one energy + gradient
 evaluation (properties pass + gradient pass) on positions already resident in
 HBM, in the mixed precision path by default.
 
+Beside the headline, the same JSON line carries blocks for device-resident
+HMC steps (`hmc`), the surrogate force network (`surrogate`), replica groups
+(`replicas`), the serial and parallel flip sweeps (`flips`), 32 L7 meshes in
+one group (`combined`) and the CPU oracle's rates (`cpu_baseline`, with its
+serial flip sweep under `flips`); each has a flag that switches it off.
+
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
