@@ -35,14 +35,8 @@ HBM_PEAK_GBPS = 8000.0
 
 
 def workload(level):
-    from trimem_amd import meshes
-    v, f, lbar = meshes.jittered_icosphere(level, seed=20261203, rel_sigma=0.01)
-    params = dict(kappa_b=30.0, kappa_a=1e6, kappa_v=1e6, kappa_c=0.0, kappa_t=1e5, kappa_r=1e3,
-                  area_frac=1.0, volume_frac=0.8, curvature_frac=1.0,
-                  bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, bond_a0=1.0, bond_r=2, bond_type=0,
-                  rep_lc1=1.5 * lbar, rep_r=2, rep_nlist=0, rep_rlist=1.5 * lbar, rep_exclusion=2,
-                  cont_delta=0.0, cont_lambda=1.0)
-    return v, f, lbar, params
+    from trimem_amd import workloads
+    return workloads.single(level)
 
 
 PROFILE_JSON = os.path.join(ROOT, "profiles", "pmc_energy_L7_mixed.json")
@@ -130,24 +124,8 @@ def cpu_flip_baseline(v, f, params, ratio=0.1, seed=7):
 
 
 def core_params(core, params):
-    """trimem_amd.core.EnergyParams of a workload() parameter dict."""
-    e = core.EnergyParams()
-    for k in ("kappa_b", "kappa_a", "kappa_v", "kappa_c", "kappa_t", "kappa_r",
-              "area_frac", "volume_frac", "curvature_frac"):
-        setattr(e, k, params[k])
-    b = core.BondParams()
-    b.type = [core.BondType.Edge, core.BondType.Area][params["bond_type"]]
-    b.r, b.lc0, b.lc1, b.a0 = params["bond_r"], params["bond_lc0"], params["bond_lc1"], params["bond_a0"]
-    e.bond_params = b
-    r = core.SurfaceRepulsionParams()
-    r.n_search = ["cell-list", "verlet-list"][params["rep_nlist"]]
-    r.rlist, r.lc1, r.r, r.exclusion_level = (params["rep_rlist"], params["rep_lc1"], params["rep_r"],
-                                              params["rep_exclusion"])
-    e.repulse_params = r
-    c = core.ContinuationParams()
-    c.delta, c.lam = params["cont_delta"], params["cont_lambda"]
-    e.continuation_params = c
-    return e
+    from trimem_amd import workloads
+    return workloads.core_params(core, params)
 
 
 def hmc_rates(v, f, params, precision, dev_steps, host_steps):
@@ -214,25 +192,8 @@ def hmc_rates(v, f, params, precision, dev_steps, host_steps):
 
 
 def replica_workload(n_replicas=64, seed=20261204):
-    """The batched-replica configuration: meshes of icosphere levels drawn
-    uniformly from 3-6 (642 to 40962 vertices), volume fractions U[0.6, 0.95],
-    kappa_b log-uniform on [10, 100] sorted into a tempering ladder; other
-    parameters as workload()."""
-    from trimem_amd import meshes
-    rng = np.random.default_rng(seed)
-    levels = rng.integers(3, 7, size=n_replicas)
-    vfrac = rng.uniform(0.6, 0.95, size=n_replicas)
-    ladder = np.sort(np.exp(rng.uniform(np.log(10.0), np.log(100.0), size=n_replicas)))
-    out = []
-    for r in range(n_replicas):
-        v, f, lbar = meshes.jittered_icosphere(int(levels[r]), seed=seed + 1 + r, rel_sigma=0.01)
-        params = dict(kappa_b=float(ladder[r]), kappa_a=1e6, kappa_v=1e6, kappa_c=0.0, kappa_t=1e5, kappa_r=1e3,
-                      area_frac=1.0, volume_frac=float(vfrac[r]), curvature_frac=1.0,
-                      bond_lc0=1.25 * lbar, bond_lc1=0.75 * lbar, bond_a0=1.0, bond_r=2, bond_type=0,
-                      rep_lc1=1.5 * lbar, rep_r=2, rep_nlist=0, rep_rlist=1.5 * lbar, rep_exclusion=2,
-                      cont_delta=0.0, cont_lambda=1.0)
-        out.append((v, f, params))
-    return out, ladder
+    from trimem_amd import workloads
+    return workloads.replicas(n_replicas, seed)
 
 
 def replica_rates(precision, evals, hmc_steps, hip, stream):
@@ -323,20 +284,11 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
     fractions U[0.55, 0.9], kappa_b log-uniform on [20, 80]; the rate of
     grouped energy+gradient evaluations of all of them and the algorithmic
     HBM rate (their working set is several times the Infinity Cache)."""
-    from trimem_amd import capi, core, meshes
+    from trimem_amd import capi, core, workloads
     L = capi.lib()
-    v0, f = meshes.icosphere(7)
-    f = np.ascontiguousarray(f.astype(np.int32))
-    lbar = meshes.mean_edge_length(v0, f)
-    rng = np.random.default_rng(seed)
-    vfrac = rng.uniform(0.55, 0.9, size=n_replicas)
-    kb = np.sort(np.exp(rng.uniform(np.log(20.0), np.log(80.0), size=n_replicas)))
-    base = workload(1)[3]    # the headline parameters, scaled to this mesh's edge below
+    work, _ = workloads.combined(n_replicas, seed)
     ms, es = [], []
-    for r in range(n_replicas):
-        v = np.ascontiguousarray(v0 + np.random.default_rng(seed + 1 + r).normal(0.0, 0.01 * lbar, v0.shape))
-        params = dict(base, kappa_b=float(kb[r]), volume_frac=float(vfrac[r]), bond_lc0=1.25 * lbar,
-                      bond_lc1=0.75 * lbar, rep_lc1=1.5 * lbar, rep_rlist=1.5 * lbar)
+    for v, f, params in work:
         m = core.TriMesh(v, f)
         e = core.EnergyManager(m, core_params(core, params))
         e.precision = precision

@@ -1,0 +1,308 @@
+"""GPU parity on the benchmark configurations at their full sizes.
+
+Each test builds its inputs with trimem_amd.workloads, exactly as bench.py
+does, and compares the HIP path (through trimem_amd.core and the C ABI) with
+the FP64 oracle on the same inputs.  Tolerances:
+
+  * FP64 path: each of the six global properties, the energy and every
+    per-vertex property column within 1e-10 relative; gradient
+    ||g - g_ref|| / ||g_ref|| <= 1e-10;
+  * mixed path (FP32 per-face/per-edge bulk, FP64 global sums): gradient
+    <= 1e-6 relative and energy <= 1e-9 relative, plus the per-property
+    bounds in MIXED_PROPERTY_TOL (each measured error with a margin);
+  * neighbour pair sets, distances and serial-flip topology: bit-exact;
+  * replica groups: every replica's energy and gradient bit-equal to the
+    single-mesh call, and a seeded sample against the oracle at the
+    precision's tolerance above.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from trimem_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PROPS = ("area", "volume", "curvature", "bending", "tethering", "repulsion")
+# mixed path, level 7, relative error bound per global property (the
+# per-face/per-edge terms are FP32 roots and angles of FP64 edge vectors,
+# summed in FP64)
+MIXED_PROPERTY_TOL = dict(area=1e-9, volume=1e-12, curvature=1e-8, bending=1e-8, tethering=1e-6, repulsion=1e-6)
+
+
+@pytest.fixture(scope="module")
+def core():
+    import trimem_amd.core as m
+    assert m.device_count() > 0, "no GPU visible"
+    return m
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def props_vec(vp):
+    return np.array([vp.area, vp.volume, vp.curvature, vp.bending, vp.tethering, vp.repulsion])
+
+
+def vertex_properties(mesh, est):
+    import ctypes
+    from trimem_amd import capi
+    fn = capi.lib().tm_energy_vertex_properties
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    out = np.zeros((mesh.n_vertices(), 6))
+    capi.check(fn(est._handle(), mesh._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+class Pair:
+    """An oracle mesh + energy manager and a device mesh + energy manager
+    built from the same points, faces and parameters."""
+
+    def __init__(self, core, v, f, p, precision="fp64"):
+        self.om = O.Mesh(v, f)
+        self.oe = O.EnergyManager(self.om, O.Params(**p))
+        self.gm = core.TriMesh(v, f)
+        self.ge = core.EnergyManager(self.gm, W.core_params(core, p))
+        self.ge.precision = precision
+
+
+@pytest.fixture(scope="module")
+def level7():
+    v, f, lbar, p = W.single(7)
+    om = O.Mesh(v, f)
+    oe = O.EnergyManager(om, O.Params(**p))
+    ref = dict(props=oe.properties(om), energy=oe.energy(om), gradient=oe.gradient(om),
+               vprops=oe.vertex_properties(om), initial=oe.initial_props())
+    return v, f, p, om, oe, ref
+
+
+def test_level7_fp64(core, level7):
+    """The single large mesh (163842 vertices) in the FP64 path."""
+    v, f, p, om, oe, ref = level7
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, W.core_params(core, p))
+    assert rel(props_vec(ge.initial_props), ref["initial"]) < 1e-10
+    pg = props_vec(ge.properties(gm))
+    for k, name in enumerate(PROPS):
+        err = abs(pg[k] - ref["props"][k]) / max(abs(ref["props"][k]), 1e-300)
+        print(f"fp64 L7 {name}: {pg[k]:.15e} rel err {err:.2e}", flush=True)
+        assert err <= 1e-10 or (ref["props"][k] == 0 and pg[k] == 0), name
+    eg = ge.energy(gm)
+    assert abs(eg - ref["energy"]) <= 1e-10 * abs(ref["energy"])
+    err_g = rel(ge.gradient(gm), ref["gradient"])
+    print(f"fp64 L7 energy rel err {abs(eg - ref['energy']) / abs(ref['energy']):.2e} gradient {err_g:.2e}")
+    assert err_g <= 1e-10
+    vp = vertex_properties(gm, ge)
+    for k, name in enumerate(PROPS):
+        if np.any(ref["vprops"][:, k]):
+            assert rel(vp[:, k], ref["vprops"][:, k]) <= 1e-10, name
+
+
+def test_level7_mixed(core, level7):
+    """The single large mesh in the mixed path: gradient <= 1e-6, energy
+    <= 1e-9 and each global property within MIXED_PROPERTY_TOL of the FP64
+    oracle; the per-vertex property columns are printed."""
+    v, f, p, om, oe, ref = level7
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, W.core_params(core, p))
+    ge.precision = "mixed"
+    pg = props_vec(ge.properties(gm))
+    errs = {}
+    for k, name in enumerate(PROPS):
+        r = ref["props"][k]
+        errs[name] = abs(pg[k] - r) / abs(r) if r != 0 else abs(pg[k])
+        print(f"mixed L7 {name}: {pg[k]:.15e} rel err {errs[name]:.2e} (bound {MIXED_PROPERTY_TOL[name]:.0e})",
+              flush=True)
+    vp = vertex_properties(gm, ge)
+    for k, name in enumerate(PROPS):
+        if np.any(ref["vprops"][:, k]):
+            print(f"mixed L7 per-vertex {name}: rel err {rel(vp[:, k], ref['vprops'][:, k]):.2e}", flush=True)
+    eg = ge.energy(gm)
+    err_e = abs(eg - ref["energy"]) / abs(ref["energy"])
+    err_g = rel(ge.gradient(gm), ref["gradient"])
+    print(f"mixed L7 energy rel err {err_e:.2e} gradient rel err {err_g:.2e}", flush=True)
+    for name in PROPS:
+        assert errs[name] <= MIXED_PROPERTY_TOL[name], name
+    assert err_e <= 1e-9
+    assert err_g <= 1e-6
+
+
+@pytest.mark.parametrize("excl", [0, 1, 2])
+def test_level7_pair_set_bit_exact(core, level7, excl):
+    """distance_matrix of the cell list (rlist 1.5 l) on the level-7 mesh at
+    exclusion levels 0, 1 and 2: the same pairs with the same distances, bit
+    for bit, and point_distances of every 997th vertex in the same order.
+    At level 2 (the configuration's) every vertex within 1.5 l is in the
+    two-ring, so the set is empty and the repulsion property is zero."""
+    v, f, p, om, oe, _ = level7
+    q = dict(p, rep_exclusion=excl)
+    gm = core.TriMesh(v, f)
+    gnl = core.make_nlist(gm, W.core_params(core, q))
+    onl = O.NeighbourList(om, O.Params(**q))
+    rl = q["rep_rlist"]
+    do, io, jo = onl.distance_matrix(om, rl)
+    dg, ig, jg = gnl.distance_matrix(gm, rl)
+    do, io, jo = np.asarray(do), np.asarray(io), np.asarray(jo)
+    dg, ig, jg = np.asarray(dg), np.asarray(ig), np.asarray(jg)
+    print("exclusion", excl, "pairs", len(do), len(dg), flush=True)
+    assert (len(do) > 2 * len(v)) if excl < 2 else len(do) == 0
+    assert len(do) == len(dg)
+    a, b = np.lexsort((jo, io)), np.lexsort((jg, ig))
+    assert np.array_equal(io[a], ig[b]) and np.array_equal(jo[a], jg[b])
+    assert np.array_equal(do[a], dg[b])
+    for pid in range(0, len(v), 997):
+        d0, j0 = onl.point_distances(om, pid, rl)
+        d1, j1 = gnl.point_distances(gm, pid, rl)
+        assert list(j0) == list(j1)
+        assert np.array_equal(np.asarray(d0).reshape((-1, 3)),
+                              np.array([[pt.x, pt.y, pt.z] for pt in d1]).reshape((-1, 3)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_level7_serial_flip_sweep_bit_exact(core, level7, precision):
+    """One serial sweep over 10% of the edges of the level-7 mesh with an
+    injected proposal and uniform stream: accepted count, consumed uniforms
+    and faces equal the oracle's; energy afterwards within 1e-10 (FP64) or
+    1e-9 (mixed).  Flip decisions are FP64 in both precisions."""
+    v, f, p, _, _, _ = level7
+    P = Pair(core, v, f, p, precision)
+    ne = P.gm.n_edges()
+    rng = np.random.default_rng(7)
+    n = int(0.1 * ne)
+    prop = rng.integers(0, ne, n).astype(np.int32)
+    uni = rng.random(n)
+    acc_o, used_o = P.oe.flip_serial_injected(P.om, prop, uni)
+    acc_g, used_g = core.flip_injected(P.gm, P.ge, prop, uni)
+    print(precision, "L7 serial sweep", n, "proposals, accepted", acc_o, acc_g, "used", used_o, used_g, flush=True)
+    assert acc_o > 0
+    assert (acc_g, used_g) == (acc_o, used_o)
+    assert np.array_equal(P.gm.fv_indices(), P.om.fv_indices())
+    eo = P.oe.energy(P.om)
+    assert abs(P.ge.energy(P.gm) - eo) <= (1e-10 if precision == "fp64" else 1e-9) * abs(eo)
+
+
+def test_level7_parallel_sweeps_consistent(core, level7):
+    """Parallel sweeps (flip ratio 0.1) on the level-7 mesh leave a manifold
+    topology with Euler characteristic 2 whose energy and gradient match an
+    oracle rebuilt from the device faces.  The reference's pflip is
+    non-deterministic, so only the outcome is compared."""
+    v, f, p, _, _, _ = level7
+    q = dict(p, kappa_a=0.0, kappa_v=0.0)     # reference values differ between the two managers otherwise
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, W.core_params(core, q))
+    accs = [core.pflip(gm, ge, 0.1, seed=11 + k) for k in range(3)]
+    fv = gm.fv_indices()
+    print("L7 pflip accepted", accs, flush=True)
+    assert min(accs) > 0
+    nv, nf = len(v), len(fv)
+    edges = np.unique(np.sort(fv[:, [0, 1, 1, 2, 2, 0]].reshape((-1, 2)), axis=1), axis=0)
+    assert nv - len(edges) + nf == 2 and len(edges) == gm.n_edges()
+    om = O.Mesh(v, fv)
+    oe = O.EnergyManager(om, O.Params(**q))
+    assert abs(ge.energy(gm) - oe.energy(om)) <= 1e-10 * abs(oe.energy(om))
+    assert rel(ge.gradient(gm), oe.gradient(om)) <= 1e-10
+
+
+def test_level4_fp64_chain(core):
+    """The plain configuration as a chain: 100 device HMC steps (dt 1e-4, 10
+    leapfrog steps, momenta and uniforms from numpy on the host), each
+    followed by a serial flip sweep over 10% of the edges from an injected
+    stream and a neighbour-list rebuild.  After every HMC step the device
+    energies E(x0), E(xn) equal the oracle's at the same positions to 1e-10,
+    and after every sweep the faces equal the oracle's bit for bit and the
+    energy agrees to 1e-10."""
+    v, f, lbar, p = W.single(4)
+    P = Pair(core, v, f, p, "fp64")
+    hmc = core.DeviceHMC(P.gm, P.ge)
+    n, ne = P.gm.n_vertices(), P.gm.n_edges()
+    np.random.seed(20261203)
+    rng = np.random.default_rng(4)
+    nflip = int(0.1 * ne)
+    e_prev = P.oe.energy(P.om)
+    acc_moves, acc_flips = 0, 0
+    for step in range(100):
+        mom = np.random.normal(size=(n, 3))
+        u = np.random.uniform()
+        hmc.step(mass=1.0, time_step=1e-4, num_integration_steps=10, temperature=1.0, momenta=mom, uniform=u)
+        r = hmc.result()
+        x = P.gm.points().copy()
+        assert abs(r["energy0"] - e_prev) <= 1e-10 * abs(e_prev), (step, r["energy0"], e_prev)
+        P.om.x = x
+        e_now = P.oe.energy(P.om)
+        if r["accepted"]:
+            acc_moves += 1
+            assert abs(r["energy1"] - e_now) <= 1e-10 * abs(e_now), (step, r["energy1"], e_now)
+        assert abs(P.ge.energy(P.gm) - e_now) <= 1e-10 * abs(e_now), step
+        prop = rng.integers(0, ne, nflip).astype(np.int32)
+        uni = rng.random(nflip)
+        ao, uo = P.oe.flip_serial_injected(P.om, prop, uni)
+        ag, ug = core.flip_injected(P.gm, P.ge, prop, uni)
+        assert (ag, ug) == (ao, uo), step
+        assert np.array_equal(P.gm.fv_indices(), P.om.fv_indices()), step
+        acc_flips += ao
+        P.ge.update_repulsion(P.gm)
+        P.oe.update_repulsion(P.om)
+        e_prev = P.oe.energy(P.om)
+        assert abs(P.ge.energy(P.gm) - e_prev) <= 1e-10 * abs(e_prev), step
+    print(f"L4 chain: {acc_moves}/100 moves accepted, {acc_flips} flips accepted", flush=True)
+    assert acc_flips > 0
+    assert rel(P.ge.gradient(P.gm), P.oe.gradient(P.om)) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_replica_set_64(core, precision):
+    """The batched-replica set (64 meshes, levels 3-6): one grouped
+    evaluation gives every replica the bits of its single-mesh energy and
+    gradient, and a seeded sample of 8 replicas matches the oracle (FP64:
+    energy and gradient 1e-10; mixed: energy 1e-9, gradient 1e-6)."""
+    work, ladder = W.replicas()
+    ms, es = [], []
+    for v, f, p in work:
+        m = core.TriMesh(v, f)
+        e = core.EnergyManager(m, W.core_params(core, p))
+        e.precision = precision
+        ms.append(m)
+        es.append(e)
+    R = core.Replicas(ms, es)
+    R.eval()
+    g = R.globals()
+    for i, (m, e) in enumerate(zip(ms, es)):
+        assert g[i, 6] == e.energy(m), i
+        assert np.array_equal(R.gradient(i), e.gradient(m)), i
+    tol_e, tol_g = (1e-10, 1e-10) if precision == "fp64" else (1e-9, 1e-6)
+    for i in np.random.default_rng(64).choice(len(work), 8, replace=False):
+        v, f, p = work[i]
+        om = O.Mesh(v, f)
+        oe = O.EnergyManager(om, O.Params(**p))
+        eo = oe.energy(om)
+        err_e, err_g = abs(g[i, 6] - eo) / abs(eo), rel(R.gradient(int(i)), oe.gradient(om))
+        print(precision, "replica", i, ms[i].n_vertices(), f"energy {err_e:.2e} gradient {err_g:.2e}", flush=True)
+        assert err_e <= tol_e and err_g <= tol_g, i
+
+
+def test_combined_set_sample(core):
+    """The combined set's 32 level-7 meshes in one group (mixed path): every
+    replica's grouped energy equals its single-mesh energy bit for bit, and
+    two seeded replicas match the oracle (energy 1e-9, gradient 1e-6)."""
+    work, _ = W.combined()
+    ms, es = [], []
+    for v, f, p in work:
+        m = core.TriMesh(v, f)
+        e = core.EnergyManager(m, W.core_params(core, p))
+        e.precision = "mixed"
+        ms.append(m)
+        es.append(e)
+    R = core.Replicas(ms, es)
+    R.eval()
+    g = R.globals()
+    for i, (m, e) in enumerate(zip(ms, es)):
+        assert g[i, 6] == e.energy(m), i
+    for i in (3, 29):
+        v, f, p = work[i]
+        om = O.Mesh(v, f)
+        oe = O.EnergyManager(om, O.Params(**p))
+        eo = oe.energy(om)
+        err_e, err_g = abs(g[i, 6] - eo) / abs(eo), rel(R.gradient(i), oe.gradient(om))
+        print("combined replica", i, f"energy {err_e:.2e} gradient {err_g:.2e}", flush=True)
+        assert err_e <= 1e-9 and err_g <= 1e-6

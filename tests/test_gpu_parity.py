@@ -533,3 +533,86 @@ def test_pflip_statistics_vs_serial(core):
     (a_s, e_s), (a_p, e_p) = stats["serial"], stats["parallel"]
     assert abs(a_p - a_s) < 0.35 * a_s, stats
     assert abs(e_p - e_s) < 0.05 * abs(e_s), stats
+
+
+# ------------------------------------------------ finite differences -------
+@pytest.mark.parametrize("bond", [0, 1])
+def test_fd_gradient(core, bond):
+    """core.gradient(mesh, estore, gradient, epsilon), the forward-difference
+    gradient of bindings.cpp:25-50, on the reference's test_gradient.py
+    setup (icosphere(3), kappas 1, fractions 0.8, lc0 1.15 l, lc1 0.85 l,
+    a0 = mean face area, epsilon 1e-8): within 1e-4 of the analytic gradient
+    (the reference test's bound) and within 1e-5 of the oracle's forward
+    differences, which round the same energies differently."""
+    v, f = M.icosphere(3)
+    om = O.Mesh(v, f)
+    a, l = om.avg_tri_props()
+    p = O.Params(kappa_b=1.0, kappa_a=1.0, kappa_v=1.0, kappa_c=1.0, kappa_t=1.0, area_frac=0.8,
+                 volume_frac=0.8, curvature_frac=0.8, bond_type=bond, bond_lc0=1.15 * l, bond_lc1=0.85 * l,
+                 bond_a0=a)
+    oe = O.EnergyManager(om, p)
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    fd = np.empty((gm.n_vertices(), 3))
+    core.gradient(gm, ge, fd, 1.0e-8)
+    assert np.abs(gm.points() - v).max() <= 1e-15    # each probe is undone by subtracting epsilon
+    e_an, e_fd = rel(fd, ge.gradient(gm)), rel(fd, oe.fd_gradient(om, 1.0e-8))
+    print(f"bond {bond}: fd vs analytic {e_an:.2e}, fd vs oracle fd {e_fd:.2e}", flush=True)
+    assert e_an < 1e-4
+    assert e_fd < 1e-5
+
+
+def bipyramid(n=32, h=0.3):
+    """Closed bipyramid over an n-gon: the two apexes have valence n, the
+    equator vertices 4.  Flipping an equator edge joins the two apexes."""
+    t = 2 * np.pi * np.arange(n) / n
+    v = np.concatenate([np.stack([np.cos(t), np.sin(t), np.zeros(n)], 1), [[0, 0, h], [0, 0, -h]]])
+    i = np.arange(n)
+    f = np.concatenate([np.stack([np.full(n, n), i, (i + 1) % n], 1),
+                        np.stack([np.full(n, n + 1), (i + 1) % n, i], 1)]).astype(np.int32)
+    return v, f
+
+
+def max_valence(f, nv):
+    e = np.unique(np.sort(f[:, [0, 1, 1, 2, 2, 0]].reshape((-1, 2)), axis=1), axis=0)
+    return int(np.bincount(e.reshape(-1), minlength=nv).max())
+
+
+def test_flips_keep_valence_within_ring_table(core):
+    """A flip that would give a vertex more neighbours than the ring table
+    holds (32) is refused, a deviation from the reference, which has no cap:
+    on the 32-gon bipyramid the oracle accepts the flip of an equator edge
+    with a zero uniform (both apexes reach valence 33), the HIP sweeps
+    (serial one-workgroup and windowed, parallel) refuse it and every later
+    evaluation still covers every vertex."""
+    v, f = bipyramid()
+    lbar = M.mean_edge_length(v, f)
+    # area/volume/curvature penalties off: the oracle rebuilt from the flipped
+    # faces takes its reference values from them
+    p = base_params(lbar, kappa_a=0.0, kappa_v=0.0, kappa_c=0.0, kappa_t=1.0, kappa_r=0.0)
+    for w in (0, 256):
+        gm = core.TriMesh(v, f)
+        ge = core.EnergyManager(gm, to_core_params(core, p))
+        t = ge.tuning
+        t["serial_window"] = w
+        ge.tuning = t
+        ne = gm.n_edges()
+        prop = np.arange(ne, dtype=np.int32)
+        acc, used = core.flip_injected(gm, ge, prop, np.zeros(ne))
+        fv = gm.fv_indices()
+        print("window", w, "accepted", acc, "used", used, "max valence", max_valence(fv, len(v)), flush=True)
+        assert max_valence(fv, len(v)) <= 32
+        om2 = O.Mesh(v, fv)
+        oe2 = O.EnergyManager(om2, p)
+        assert abs(ge.energy(gm) - oe2.energy(om2)) <= 1e-10 * abs(oe2.energy(om2))
+    # the oracle (no cap) accepts flipping an equator edge of the original mesh
+    om3 = O.Mesh(v, f)
+    oe3 = O.EnergyManager(om3, p)
+    ne = om3.n_edges()
+    acc_o, _ = oe3.flip_serial_injected(om3, np.arange(ne, dtype=np.int32), np.zeros(ne))
+    assert acc_o > 0 and max_valence(om3.fv_indices(), len(v)) > 32
+    gm = core.TriMesh(v, f)
+    ge = core.EnergyManager(gm, to_core_params(core, p))
+    for k in range(4):
+        core.pflip(gm, ge, 1.0, seed=50 + k)
+        assert max_valence(gm.fv_indices(), len(v)) <= 32

@@ -273,3 +273,28 @@ def test_grouped_pair_cache_is_bit_identical(core):
     on.hmc_run(3)
     off.hmc_run(3)
     same("after rebuild and hmc")
+
+
+def test_member_rebuild_refreshes_group(core):
+    """A member's own update_repulsion (the reference callback's path,
+    evaluators.py:85) swaps in a new neighbour list; the group notices and
+    evaluates with the member's current list: its energies and gradients stay
+    bit-equal to the single-mesh calls, also after the member's list buffers
+    grow, and a captured HMC step is recaptured against the new table."""
+    reps = [membrane(core, 3, 41, "fp64"), membrane(core, 3, 42, "fp64", excl=2)]
+    R = core.Replicas([m for m, _ in reps], [e for _, e in reps])
+    R.hmc_setup(seeds=[1, 2], mass=1.0, time_step=1e-3, temperature=1.0, num_integration_steps=4)
+    R.hmc_run(2, use_graph=True)
+    m0, e0 = reps[0]
+    x = m0.points()
+    x *= 1.5                                     # a larger box: the cell grid grows
+    e0.update_repulsion(m0)
+    R.eval()
+    g = R.globals()
+    for i, (m, e) in enumerate(reps):
+        assert g[i, 6] == e.energy(m), i
+        assert np.array_equal(R.gradient(i), e.gradient(m)), i
+    R.hmc_run(2, use_graph=True)                 # replays against the refreshed table
+    for i, (m, e) in enumerate(reps):
+        R.eval()
+        assert R.globals()[i, 6] == e.energy(m), i

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtrimem_hip.so")
+OBJ_DIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["tm_capi.hip", "tm_energy.hip", "tm_nlist.hip", "tm_flips.hip", "tm_hmc.hip", "tm_surrogate.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -39,9 +40,19 @@ def _run(cmd, cwd=None):
 
 
 def build_lib():
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Each source compiles to its own object in parallel (no device code is
+    shared between files except through inline header functions), then one
+    link."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    cmds = [[HIPCC] + compile_flags + ["-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(HIP_SOURCES, objs)]
+    jobs = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
 
 

@@ -911,7 +911,7 @@ PYBIND11_MODULE(core, m)
                 }
             }
         },
-        py::arg("mesh"), py::arg("estore"), py::arg("gradient"), py::arg("epsilon"));
+        py::arg("mesh"), py::arg("estore"), py::arg("gradient"), py::arg("epsilon") = 1.0e-6);
 
     m.def("device_count", []() { int n = 0; check(tm_device_count(&n)); return n; });
 }

@@ -593,6 +593,9 @@ struct tm_energy {
     DevBuf<unsigned long long> sf_vstamp, sf_estamp, sf_fstamp;
     Tuning tune;
     tm_nlist* spare = nullptr;    // the previous list, rebuilt in place by update_repulsion
+    // bumped whenever something a replica group copies into its device
+    // table changes: the neighbour list, the reference values, the precision
+    unsigned gen = 0;
     ~tm_energy()
     {
         delete nlist;
@@ -651,6 +654,7 @@ void rebuild_nlist(tm_energy& E, tm_mesh& M)
     L.rlist = E.params.rep_rlist;
     build_nlist(L, M);
     std::swap(E.nlist, E.spare);
+    E.gen++;
 }
 
 // The pair-cache buffers of the energy manager's cell list (x_snap null when
@@ -971,6 +975,7 @@ int tm_energy_set_precision(tm_energy* em, int precision)
 {
     API_BEGIN
     require(precision == TM_PREC_FP64 || precision == TM_PREC_MIXED, "unknown precision");
+    if (em->precision != precision) em->gen++;
     em->precision = precision;
     API_END
 }
@@ -1044,7 +1049,10 @@ int tm_energy_energy_gradient(tm_energy* em, tm_mesh* mesh, double* energy, doub
 int tm_energy_update_reference_properties(tm_energy* em)
 {
     // energy.cpp:47-55
-    if (em->params.cont_lambda < 1.0) em->params.cont_lambda += em->params.cont_delta;
+    if (em->params.cont_lambda < 1.0) {
+        em->params.cont_lambda += em->params.cont_delta;
+        em->gen++;
+    }
     return TM_OK;
 }
 
@@ -1458,6 +1466,7 @@ struct tm_replicas {
     DevBuf<double> disp_parts;
     std::vector<int> h_ones;
     int total_disp = 0, total_rc = 0;
+    std::vector<unsigned> gens;    // each member's tm_energy::gen when the table was built
     GroupLaunch group() const
     {
         return GroupLaunch{d_args.p,  h_args.data(), R, d_block0.p, total_blocks, d_block0_rep.p, total_rep,
@@ -1548,6 +1557,8 @@ void refresh_replica_args(tm_replicas& S)
         S.h_block0[r + 1] = S.h_block0[r] + S.h_args[r].nblocks;
     }
     S.total_blocks = S.h_block0[S.R];
+    S.gens.resize(S.R);
+    for (int r = 0; r < S.R; r++) S.gens[r] = S.ems[r]->gen;
     S.d_args.alloc(S.R);
     S.d_block0.alloc(S.R + 1);
     h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
@@ -1584,10 +1595,30 @@ void replicas_device_points(tm_replicas& S)
         if (!M->device_ahead) push_points(*M);
 }
 
+// The device table current for every member: a member whose manager
+// rebuilt its neighbour list (update_repulsion swaps in the spare list),
+// advanced its continuation or changed precision since the table was built
+// makes the group rebuild the table and drop its captured graph, whose
+// launches read the old list.  A precision different from the group's is
+// refused.
+void replicas_current(tm_replicas& S)
+{
+    bool stale = false;
+    for (int r = 0; r < S.R; r++) {
+        if (S.ems[r]->precision != S.mixed)
+            throw TmError(TM_E_INVALID, "a replica's precision differs from the group's");
+        stale = stale || S.ems[r]->gen != S.gens[r];
+    }
+    if (!stale) return;
+    S.drop_graph();
+    refresh_replica_args(S);
+}
+
 // exclusion masks current for every replica's topology (the table holds
 // the mask buffers, whose addresses do not change)
 void replicas_masks(tm_replicas& S)
 {
+    replicas_current(S);
     for (int r = 0; r < S.R; r++) {
         tm_nlist& L = *S.ems[r]->nlist;
         const unsigned before = L.mask_topo;
@@ -1764,6 +1795,7 @@ int tm_replicas_swap(tm_replicas* rs)
     API_BEGIN
     tm_replicas& S = *rs;
     require(S.has_ladder && S.hmc_ready, "exchange needs a ladder and HMC parameters");
+    replicas_current(S);
     launch_pt_swap(S.d_args.p, S.d_hmc.p, S.owner.p, S.ladder.p, S.R, S.pt_seed, S.pt_round.p, S.swaps_acc.p,
                    S.swaps_tried.p, g_rt.stream);
     check_launch();

@@ -79,21 +79,35 @@ __device__ __forceinline__ int rng_edge(uint64_t seed, uint64_t i, int ne)
 }
 
 // ------------------------------------------------------------ helpers -----
+// OpenMesh TriConnectivity::is_flip_ok (not a boundary edge, the two
+// opposite vertices distinct and not yet neighbours), plus one condition the
+// reference does not have: a flip adds a neighbour to each opposite vertex,
+// and it is refused when either would exceed the ring-table width
+// kMaxValence.  Meshes are created with valences of at most kMaxValence, so
+// every topology a sweep can reach stays within the table.  The two one-rings
+// are walked in one loop so that their dependent loads overlap.
 __device__ __forceinline__ bool flip_ok(const Topo& t, int e)
 {
-    // OpenMesh TriConnectivity::is_flip_ok
     if (t.edge_boundary(e)) return false;
     int ah = t.to(t.next(2 * e));
     int bh = t.to(t.next(2 * e + 1));
     if (ah == bh) return false;
-    int h0 = t.vhe(ah);
-    int h = h0;
-    int guard = 0;
+    const int ha0 = t.vhe(ah), hb0 = t.vhe(bh);
+    int ha = ha0, hb = hb0, na = 0, nb = 0;
+    bool da = false, db = false, adjacent = false;
     do {
-        if (t.to(h) == bh) return false;
-        h = t.cw(h);
-    } while (h != h0 && ++guard < 4 * kMaxValence);
-    return true;
+        if (!da) {
+            adjacent = adjacent || t.to(ha) == bh;
+            ha = t.cw(ha);
+            da = ha == ha0 || ++na >= 4 * kMaxValence;
+        }
+        if (!db) {
+            hb = t.cw(hb);
+            db = hb == hb0 || ++nb >= 4 * kMaxValence;
+        }
+    } while (!(da && db));
+    // na, nb count the steps before the walk closed: valence - 1
+    return !adjacent && na + 1 < kMaxValence && nb + 1 < kMaxValence;
 }
 
 __device__ __forceinline__ void do_flip(const FlipMeshView& fm, int e)
