@@ -65,7 +65,7 @@ def test_velocity_verlet_harmonic_map(dt, n):
     [[1 - h^2/2, h], [-h (1 - h^2/4), 1 - h^2/2]] on (x, p)."""
     rng = np.random.default_rng(3)
     x0, p0 = rng.normal(size=(5, 3)), rng.normal(size=(5, 3))
-    x, p = H._vv_integration(x0, p0, lambda y: -y, 1.0, dt, n)
+    x, p = H.leapfrog(x0, p0, lambda y: -y, 1.0, dt, n)
     A = np.array([[1 - dt**2 / 2, dt], [-dt * (1 - dt**2 / 4), 1 - dt**2 / 2]])
     An = np.linalg.matrix_power(A, n)
     assert np.allclose(x, An[0, 0] * x0 + An[0, 1] * p0, rtol=0, atol=1e-12)
@@ -81,7 +81,7 @@ def test_velocity_verlet_mass_and_order():
     force = lambda y: -y
     errs = []
     for dt in (0.02, 0.01):
-        x, p = H._vv_integration(x0, p0, force, 2.0, dt, int(round(1.0 / dt)))
+        x, p = H.leapfrog(x0, p0, force, 2.0, dt, int(round(1.0 / dt)))
         errs.append(abs((0.5 * (x * x).sum() + 0.25 * (p * p).sum()) - (0.5 + 0.25 * 0.25)))
     assert 3.5 < errs[0] / errs[1] < 4.5
 

@@ -1529,7 +1529,7 @@ void refresh_replica_args(tm_replicas& S)
     std::vector<int> b0d(S.R + 1, 0), b0c(S.R + 1, 0);
     for (int r = 0; r < S.R; r++) {
         b0d[r + 1] = b0d[r] + std::min(std::max((S.meshes[r]->nv + 4095) / 4096, 1), 64);
-        b0c[r + 1] = b0c[r] + pass1_blocks(S.meshes[r]->nv) / 8;
+        b0c[r + 1] = b0c[r] + (S.meshes[r]->nv + kCachedVerts - 1) / kCachedVerts;
     }
     S.total_disp = b0d[S.R];
     S.total_rc = (b0c[S.R] + 7) / 8 * 8;    // a multiple of the 8 XCDs; padding blocks fall past the last replica's vertices
@@ -1574,7 +1574,7 @@ void set_group_shape(tm_replicas& S, const Tuning& t)
     std::vector<int> rep(S.R + 1, 0), p2(S.R + 1, 0);
     for (int r = 0; r < S.R; r++) {
         rep[r + 1] = rep[r] + tuned_blocks(S.meshes[r]->nv, t.rep_threads);
-        p2[r + 1] = p2[r] + tuned_blocks(S.meshes[r]->nv, t.p2_threads);
+        p2[r + 1] = p2[r] + pass2_blocks(S.meshes[r]->nv, t.p2_threads);
     }
     S.tune = t;
     S.tune.rep_fused = 0;     // grouped launches keep the query apart
@@ -1927,13 +1927,10 @@ double time_launches(F&& f, int reps)
 }
 
 // Try every shape of the repulsion query and of pass 2 with launch(t) and
-// keep the fastest of each; the two kernels are timed separately.  With
-// try_fused, the pass-1 launches (query + properties) of the best separate
-// shape are then timed against the fused launch at every batch, and the
-// faster form is kept (best_rep_ms is then the fused launch's time).
-template <class RepF, class P2F, class P1F>
-Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, P1F&& p1, bool try_fused, double* best_rep_ms,
-              double* best_p2_ms)
+// keep the fastest of each; the two kernels are timed separately.  The query
+// always has its own launch (rep_fused is kept at 0).
+template <class RepF, class P2F>
+Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, double* best_rep_ms, double* best_p2_ms)
 {
     const int ths[4] = {64, 128, 256, 512}, bats[4] = {1, 2, 4, 8};
     Tuning best = base;
@@ -1953,26 +1950,6 @@ Tuning search(Tuning base, bool has_rep, RepF&& rep, P2F&& p2, P1F&& p1, bool tr
                 }
             }
     best.rep_fused = 0;
-    if (has_rep && try_fused) {
-        const double apart = time_launches([&] { p1(best); }, 5);
-        double bf = 1e30;
-        int bb = best.rep_batch;
-        for (int ba : bats) {
-            Tuning t = best;
-            t.rep_fused = 1;
-            t.rep_batch = ba;
-            const double ms = time_launches([&] { p1(t); }, 5);
-            if (ms < bf) {
-                bf = ms;
-                bb = ba;
-            }
-        }
-        if (bf < apart) {
-            best.rep_fused = 1;
-            best.rep_batch = bb;
-            brep = bf;
-        }
-    }
     for (int th : ths) {
         Tuning t = best;
         t.p2_threads = th;
@@ -2019,9 +1996,7 @@ int tm_energy_autotune(tm_energy* em, tm_mesh* mesh, tm_tuning* out, double* ms_
     if (M.nv > 0)
         E.tune = search(
             E.tune, a.rep_v != nullptr, [&](const Tuning& t) { launch_rep(a, t, g_rt.stream); },
-            [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); },
-            [&](const Tuning& t) { launch_pass1_body(a, E.precision, t, g_rt.stream); },
-            a.pc.x_snap == nullptr, ms_rep, ms_pass2);    // with the cache the query is never fused
+            [&](const Tuning& t) { launch_pass2_kernel(a, E.precision, t, g_rt.stream); }, ms_rep, ms_pass2);
     if (out) *out = to_c(E.tune);
     API_END
 }
@@ -2060,8 +2035,7 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
     };
     const Tuning best = search(
         S.tune, has_rep, [&](const Tuning& t) { launch_rep_grouped(shaped(t), g_rt.stream); },
-        [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, [](const Tuning&) {}, false, ms_rep,
-        ms_pass2);
+        [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, ms_rep, ms_pass2);
     S.drop_graph();
     set_group_shape(S, best);
     if (out) *out = to_c(best);

@@ -89,6 +89,29 @@ struct Prec<float> {
     }
 };
 
+// The edge tether (mesh_tether.h:40-82) amplifies a length error near its
+// walls: exp(l / (l - lc1)) and (l - lc0)^r make dV/V ~ l lc1 / (l - lc1)^2
+// x dl/l, hundreds of times the rounding of an FP32 length at the
+// configurations' bond widths.  So both precisions evaluate it from the FP64
+// length; the potential is zero between the walls, so the FP64 root is only
+// taken for edges outside them (l2: the FP64 squared length).
+__device__ __forceinline__ bool tether_active(double l2, const EParams& p)
+{
+    return l2 > p.bond_lc0 * p.bond_lc0 || l2 < p.bond_lc1 * p.bond_lc1;
+}
+template <class T>
+__device__ __forceinline__ double tether_value(double l2, T l, const EParams& p)
+{
+    if constexpr (sizeof(T) == sizeof(double)) return bond_edge_value((double)l, p);
+    else return tether_active(l2, p) ? bond_edge_value(sqrt(l2), p) : 0.0;
+}
+template <class T>
+__device__ __forceinline__ double tether_dfac(double l2, T l, const EParams& p)
+{
+    if constexpr (sizeof(T) == sizeof(double)) return bond_edge_dfac((double)l, p);
+    else return tether_active(l2, p) ? bond_edge_dfac(sqrt(l2), p) : 0.0;
+}
+
 __device__ __forceinline__ d3 load_p(const double* __restrict__ x, int v)
 {
     return d3{x[3 * v], x[3 * v + 1], x[3 * v + 2]};
@@ -222,34 +245,6 @@ __device__ __forceinline__ int group_sum(int x)
     x += dpp_i<kDppXor2>(x);
     x += dpp_i<kDppHalfMirror>(x);
     return x;
-}
-
-// Block sum of six doubles held by the group leaders (lane % 8 == 0); the
-// result is in s_out[0..5].
-__device__ __forceinline__ void block_sum6_leaders(double v[6], double* s_red, double* s_out)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        double a = v[k];
-        a += __shfl_xor(a, 8, 64);
-        a += __shfl_xor(a, 16, 64);
-        a += __shfl_xor(a, 32, 64);
-        v[k] = a;
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; k++) s_red[wave * 6 + k] = v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        double a = 0;
-        for (int w = 0; w < nw; w++) a += s_red[w * 6 + threadIdx.x];
-        s_out[threadIdx.x] = a;
-    }
-    __syncthreads();
 }
 
 // Exclusion test of the cell list (cell_list.h:170-202) on the ring table:
@@ -633,47 +628,63 @@ __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e
                                                    double& area, double& vol, double& curv, double& teth)
 {
     T af = Prec<T>::sqrt_(dot_f(n_cur, n_cur)) * (T)0.5;
-    T l = Prec<T>::sqrt_(dot_f(e_cur, e_cur));
+    const double l2 = dot_f(e_cur, e_cur);
+    T l = Prec<T>::sqrt_(l2);
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     curv += (double)((T)0.5 * l * th);
     area += (double)af;
     vol += dot_f(xv, n_cur);    // 6 x the signed volume x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
-    if (p.bond_type == 0) teth += bond_edge_value((double)l, p);
+    if (p.bond_type == 0) teth += tether_value<T>(l2, l, p);
     else teth += bond_area_value((double)af, p);
 }
 
+// One lane per vertex.  Lane v walks the clockwise ring of v once, carrying
+// the spoke e_k and the sector normal n_k from one slot to the next, so every
+// normal of the ring is formed once, and the per-vertex work (the sums, the
+// bending ratio, the stores) runs once per vertex instead of once per lane of
+// a lane group.  Column k of the ring table holds the k-th neighbours of all
+// vertices, so each slot's load is coalesced across the wave; columns below
+// kRingPad hold -1 past the valence, so a ring of at most kRingPad slots is
+// loaded in full before its valence is known, and its position loads issue
+// together.  Longer rings walk the table slot by slot.
 template <class T>
-__device__ __forceinline__ void slot_properties(const MeshView& m, const EParams& p, int v, int n, int k, d3 xv,
+__device__ __forceinline__ void ring_properties(const MeshView& m, const EParams& p, int v, int n, d3 xv,
                                                 double& area, double& vol, double& curv, double& teth)
 {
     const int N = m.nv;
-    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
-    const d3 e_prev = load_p(m.x, m.ring[km * N + v]) - xv;
-    const d3 e_cur = load_p(m.x, m.ring[k * N + v]) - xv;
-    const d3 e_next = load_p(m.x, m.ring[kp * N + v]) - xv;
-    const d3 n_cur = cross_f(e_cur, e_prev);
-    const d3 n_next = cross_f(e_next, e_cur);
-    slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
-}
-
-__device__ __forceinline__ d3 shfl_d3(d3 a, int src)
-{
-    return d3{__shfl(a.x, src, 64), __shfl(a.y, src, 64), __shfl(a.z, src, 64)};
-}
-
-// Rings of at most kGroup: lane k of the group owns slot k and loads only
-// its own neighbour; e_{k-1} comes from lane k-1 and n_{k+1} from lane k+1
-// (the values those lanes computed, so the bits are those of
-// slot_properties).  Lanes n..kGroup-1 add nothing, and no lane below n
-// reads them.
-struct RingLanes {
-    int k, src_prev, src_next;
-};
-__device__ __forceinline__ RingLanes ring_lanes(int n, int lane)
-{
-    const int k = lane < n ? lane : 0;
-    const int gl = (int)(threadIdx.x & 63) - lane;    // first lane of the group in the wave
-    return RingLanes{k, gl + (k == 0 ? n - 1 : k - 1), gl + (k + 1 == n ? 0 : k + 1)};
+    if (n <= kRingPad) {
+        d3 e[kRingPad];
+#pragma unroll
+        for (int k = 0; k < kRingPad; k++) {
+            const int j = m.ring[k * N + v];
+            e[k] = load_p(m.x, j >= 0 ? j : v) - xv;    // past the valence: x_v, never used
+        }
+        d3 e_last = e[kRingPad - 1];
+#pragma unroll
+        for (int k = kRingPad - 2; k >= 2; k--)
+            if (k == n - 1) e_last = e[k];
+        d3 n_cur = cross_f(e[0], e_last);
+#pragma unroll
+        for (int k = 0; k < kRingPad; k++) {
+            if (k < n) {
+                const d3 e_next = k + 1 < n ? e[(k + 1) % kRingPad] : e[0];
+                const d3 n_next = cross_f(e_next, e[k]);
+                slot_properties_at<T>(p, xv, e[k], n_cur, n_next, area, vol, curv, teth);
+                n_cur = n_next;
+            }
+        }
+        return;
+    }
+    const d3 e_first = load_p(m.x, m.ring[v]) - xv;
+    d3 e_cur = e_first;
+    d3 n_cur = cross_f(e_first, load_p(m.x, m.ring[(n - 1) * N + v]) - xv);
+    for (int k = 0; k < n; k++) {
+        const d3 e_next = k + 1 < n ? load_p(m.x, m.ring[(k + 1) * N + v]) - xv : e_first;
+        const d3 n_next = cross_f(e_next, e_cur);
+        slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
+        e_cur = e_next;
+        n_cur = n_next;
+    }
 }
 
 __device__ __forceinline__ bool is_generic(int meta)
@@ -681,9 +692,6 @@ __device__ __forceinline__ bool is_generic(int meta)
     return meta_bslot(meta) >= 0 || meta_valence(meta) < 3;
 }
 
-// Pass 1 over all vertices; vertices on the boundary (or with fewer than
-// three neighbours) read the properties that k_generic_props wrote.  Writes
-// q_v, the number of repulsion pairs of v, and the FP64 globals.
 // replica of logical block lb in a grouped launch (block0 ascending, block0[0] = 0)
 __device__ __forceinline__ int find_replica(const int* __restrict__ block0, int R, int lb)
 {
@@ -696,110 +704,60 @@ __device__ __forceinline__ int find_replica(const int* __restrict__ block0, int 
     return lo;
 }
 
-// kRepApart: the repulsion property (column 5 of the block partial and of
-// vprops) comes from a block of k_pass1_rep running beside this one, which
-// sums it in the same order as below; this block leaves that column alone.
-template <class T, bool kRepApart = false>
+// Pass 1 over the kPass1Verts vertices of block lb; vertices on the boundary
+// (or with fewer than three neighbours) read the properties that
+// k_generic_props wrote.  The repulsion property comes from the query kernel
+// that ran before (rep_v).  Writes q_v, optionally the per-vertex properties,
+// and the block's partial sums of the six globals (FP64, a fixed butterfly
+// order, so the result does not depend on timing).
+template <class T>
 __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
     const EParams& p = A.p;
-    __shared__ double s_red[4 * 6];
+    __shared__ double s_red[(kPass1Verts / 64) * 6];
     __shared__ double s_out[6];
-    const int t = lb * blockDim.x + threadIdx.x;
-    const int v = t / kGroup, lane = t % kGroup;
+    const int v = lb * kPass1Verts + threadIdx.x;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
-    // repulsion sum of v from k_rep, loaded alongside meta
-    const double rv = (!kRepApart && A.rep_v && active && lane == 0) ? A.rep_v[v] : 0.0;
+    const double rv = (A.rep_v && active) ? A.rep_v[v] : 0.0;
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
-    double area = 0, vol = 0, curv = 0, teth = 0;
-    // ring column `lane` needs no valence (columns up to kGroup hold -1 past
-    // it), so it loads beside meta instead of after it
-    const int jl = active ? m.ring[lane * m.nv + v] : -1;
-    if (ring) {
-        const d3 xv = load_p(m.x, v);
-        if (n <= kGroup) {
-            const RingLanes r = ring_lanes(n, lane);
-            // lanes past the valence take v itself; no lane below it reads them
-            const d3 e_cur = load_p(m.x, jl >= 0 ? jl : v) - xv;
-            const d3 e_prev = shfl_d3(e_cur, r.src_prev);
-            const d3 n_cur = cross_f(e_cur, e_prev);
-            const d3 n_next = shfl_d3(n_cur, r.src_next);
-            if (lane < n) slot_properties_at<T>(p, xv, e_cur, n_cur, n_next, area, vol, curv, teth);
-        } else {
-            for (int k = lane; k < n; k += kGroup) slot_properties<T>(m, p, v, n, k, xv, area, vol, curv, teth);
-        }
-    }
-    area = group_sum(area);
-    vol = group_sum(vol);
-    curv = group_sum(curv);
-    teth = group_sum(teth);
     double pr[6] = {0, 0, 0, 0, 0, 0};
-    __shared__ double s_q[256 / kGroup];
-    if (active && lane == 0) {
-        double q;    // curvature / area
-        if (ring) {
-            // a third of the face areas and volumes, half the edge terms (by
-            // the rounded reciprocals: at most the last bit differs from a
-            // division, far inside the parity tolerance)
-            const double a = area * (1.0 / 3.0), c = curv * 0.5;
-            q = c / a;
-            pr[0] = a;
-            pr[1] = vol * (1.0 / 18.0);
-            pr[2] = c;
-            pr[3] = 2 * c * q;
-            pr[4] = p.bond_type == 0 ? teth * 0.5 : teth * (1.0 / 3.0);
-            pr[5] = rv;
-        } else {
-            for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
-            q = pr[2] / pr[0];
-        }
-        if (kRepApart) pr[5] = 0.0;
-        s_q[threadIdx.x / kGroup] = q;
-        if (A.vprops) {
-            for (int k = 0; k < (kRepApart ? 5 : 6); k++) A.vprops[6 * v + k] = pr[k];
-        }
+    if (ring) {
+        double area = 0, vol = 0, curv = 0, teth = 0;
+        ring_properties<T>(m, p, v, n, load_p(m.x, v), area, vol, curv, teth);
+        // a third of the face areas and volumes, half the edge terms (by
+        // the rounded reciprocals: at most the last bit differs from a
+        // division, far inside the parity tolerance)
+        const double a = area * (1.0 / 3.0), c = curv * 0.5;
+        const double q = c / a;    // curvature / area
+        pr[0] = a;
+        pr[1] = vol * (1.0 / 18.0);
+        pr[2] = c;
+        pr[3] = 2 * c * q;
+        pr[4] = p.bond_type == 0 ? teth * 0.5 : teth * (1.0 / 3.0);
+        pr[5] = rv;
+        A.q[v] = q;
+    } else if (active) {
+        for (int k = 0; k < 6; k++) pr[k] = A.gprops[6 * v + k];
+        A.q[v] = pr[2] / pr[0];
     }
-    block_sum6_leaders(pr, s_red, s_out);    // its barriers also publish s_q
-    {
-        // the block's 32 consecutive vertices, written contiguously
-        const int v0 = (lb * blockDim.x) / kGroup;
-        const int i = threadIdx.x;
-        if (i < 256 / kGroup && v0 + i < m.nv) A.q[v0 + i] = s_q[i];
-    }
-    if (threadIdx.x < (kRepApart ? 5 : 6)) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
+    if (A.vprops && active)
+        for (int k = 0; k < 6; k++) A.vprops[6 * v + k] = pr[k];
+    block_sum6(pr, s_red, s_out);
+    if (threadIdx.x < 6) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_pass1(PassArgs A)
+__global__ void __launch_bounds__(kPass1Verts) k_pass1(PassArgs A)
 {
     pass1_body<T>(A, logical_block());
 }
 
-// pass 1 without the repulsion column (written by the query's kernel)
 template <class T>
-__global__ void __launch_bounds__(256) k_pass1_apart(PassArgs A)
-{
-    pass1_body<T, true>(A, logical_block());
-}
-
-// replica groups with the cached query: replicas with repulsion take their
-// repulsion column from it
-template <class T>
-__global__ void __launch_bounds__(256) k_pass1_apart_grouped(const PassArgs* __restrict__ A,
-                                                             const int* __restrict__ block0, int R)
-{
-    const int lb = logical_block();
-    const int r = find_replica(block0, R, lb);
-    if (A[r].rep_v) pass1_body<T, true>(A[r], lb - block0[r]);
-    else pass1_body<T, false>(A[r], lb - block0[r]);
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
-                                                       int R)
+__global__ void __launch_bounds__(kPass1Verts) k_pass1_grouped(const PassArgs* __restrict__ A,
+                                                               const int* __restrict__ block0, int R)
 {
     const int lb = logical_block();
     const int r = find_replica(block0, R, lb);
@@ -813,14 +771,9 @@ __global__ void __launch_bounds__(256) k_pass1_grouped(const PassArgs* __restric
 // chain of dependent loads (cell, candidate id, position, ring of the
 // candidate) and needs few registers, so it runs apart from the
 // register-heavy ring work of pass 1.
-// kPartial (256 threads, the 32 vertices of pass-1 block lb): also write
-// column 5 of that block's partial and of vprops, summed over the block's
-// group leaders exactly as pass1_body sums it, so the bits do not depend on
-// which of the two kernels computes it.
-template <int kThreads, int kBatch, bool kPartial = false>
+template <int kThreads, int kBatch>
 __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
 {
-    static_assert(!kPartial || kThreads == 256, "partials are per 256-thread pass-1 block");
     if (!A.rep_v) return;
     const MeshView& m = A.m;
     const EParams& p = A.p;
@@ -850,17 +803,6 @@ __device__ __forceinline__ void rep_body(const PassArgs& A, int lb)
     if (i < kThreads / kGroup && v0 + i < m.nv) {
         A.rep_v[v0 + i] = s_rep[i];
         A.pair_cnt[v0 + i] = s_cnt[i];
-    }
-    if constexpr (kPartial) {
-        __shared__ double s_red[4 * 6];
-        __shared__ double s_out[6];
-        double pr[6] = {0, 0, 0, 0, 0, 0};
-        if (active && lane == 0) {
-            pr[5] = ring ? rep : A.gprops[6 * v + 5];
-            if (A.vprops) A.vprops[6 * v + 5] = pr[5];
-        }
-        block_sum6_leaders(pr, s_red, s_out);
-        if (threadIdx.x == 5) A.partials[5 * A.nblocks + lb] = s_out[5];
     }
 }
 
@@ -1035,7 +977,6 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
     const unsigned rec_word = mine && pc.x_snap ? pc.cnt[v] : 0u;
     const int mode = pc.x_snap ? pair_cache_mode(pc) : 0;    // block-uniform: 1 replay, 2 record, 0 full
     __shared__ int s_list[kVertsPerBlock];
-    __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (pr[5] of pass 1)
     __shared__ int s_n;
     __shared__ int s_rest[kVertsPerBlock];
     __shared__ int s_nrest;
@@ -1044,7 +985,6 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         s_nrest = 0;
     }
     __syncthreads();
-    if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
     if (mine) {
         const bool ring = !is_generic(meta);
         bool done = !ring;
@@ -1096,7 +1036,6 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         if (done) {
             A.rep_v[v] = rep;
             A.pair_cnt[v] = cnt;
-            s_val[threadIdx.x] = ring ? rep : A.gprops[6 * v + 5];
         } else {
             s_list[atomicAdd(&s_n, 1)] = v;
         }
@@ -1118,7 +1057,6 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
                 if ((threadIdx.x & 63) == 0) {
                     A.rep_v[u] = tr.rep;
                     A.pair_cnt[u] = tr.cnt;
-                    s_val[u - base] = tr.rep;
                     store_rep_g(A, u, tr);
                 }
             } else if ((threadIdx.x & 63) == 0) {
@@ -1145,27 +1083,8 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         if (lane == 0) {
             A.rep_v[u] = tr.rep;
             A.pair_cnt[u] = tr.cnt;
-            s_val[u - base] = tr.rep;
             store_rep_g(A, u, tr);
         }
-    }
-    __syncthreads();
-    if (A.vprops && threadIdx.x < kVertsPerBlock && v < m.nv) A.vprops[6 * v + 5] = s_val[threadIdx.x];
-    // column 5 of the pass-1 partials of the block's 32-vertex chunks, summed
-    // as block_sum6_leaders sums it in pass 1: within each 8-vertex wave
-    // ((g0+g1)+(g2+g3))+((g4+g5)+(g6+g7)), then the four waves in order
-    constexpr int kChunks = kVertsPerBlock / 32;
-    if (threadIdx.x < kChunks) {
-        const int c = threadIdx.x;
-        const double* g = s_val + 32 * c;
-        double a = 0.0;
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            const double* q = g + 8 * w;
-            a += ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
-        }
-        const int lb1 = base / 32 + c;
-        if (lb1 < A.nblocks) A.partials[5 * A.nblocks + lb1] = a;
     }
 }
 
@@ -1239,29 +1158,6 @@ __global__ void __launch_bounds__(256) k_rep_record(PassArgs A)
         A.rep_v[v0 + i] = s_rep[i];
         A.pair_cnt[v0 + i] = s_cnt[i];
     }
-    // column 5 of pass-1 block lb (the same 32 vertices), as pass 1 sums it
-    __shared__ double s_red[4 * 6];
-    __shared__ double s_out[6];
-    double pr[6] = {0, 0, 0, 0, 0, 0};
-    if (active && lane == 0) {
-        pr[5] = ring ? rep : A.gprops[6 * v + 5];
-        if (A.vprops) A.vprops[6 * v + 5] = pr[5];
-    }
-    block_sum6_leaders(pr, s_red, s_out);
-    if (threadIdx.x == 5) A.partials[5 * A.nblocks + lb] = s_out[5];
-}
-
-// Pass 1 and the repulsion query in one launch of 2 x nblocks blocks: logical
-// block 2b runs the query for the 32 vertices of pass-1 block b, 2b + 1 their
-// ring work.  The query waits on chains of loads and the ring work on the
-// vector ALU, so the two kinds of wave fill each other's idle cycles on the
-// same compute units (the pair shares an XCD and its L2).
-template <class T, int kBatch>
-__global__ void __launch_bounds__(256) k_pass1_rep(PassArgs A)
-{
-    const int lb = logical_block();
-    if (lb & 1) pass1_body<T, true>(A, lb >> 1);
-    else rep_body<256, kBatch, true>(A, lb >> 1);
 }
 
 template <int kThreads, int kBatch>
@@ -1385,27 +1281,81 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     acc = axpy_f(g.fac_v6, n_cur, acc);
     acc = axpy_f(kb * wq, u, acc);
     // spoke (v, j_k) with faces F_k and F_{k+1}
-    T l = Prec<T>::sqrt_(dot_f(e_cur, e_cur));
+    const double l2 = dot_f(e_cur, e_cur);
+    T l = Prec<T>::sqrt_(l2);
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     double fe = -(kb * (qv + q_cur) + 0.5 * g.fac_c) * (double)th;
-    if (p.bond_type == 0) fe -= kt * bond_edge_dfac((double)l, p);
+    if (p.bond_type == 0) fe -= kt * tether_dfac<T>(l2, l, p);
     return axpy_f(fe * (double)Prec<T>::rcp(l), e_cur, acc);
 }
 
+// The gradient at interior vertex v, one lane per vertex, walking the ring
+// as pass 1 does (ring_properties): slot k needs the spokes e_{k-1}, e_k,
+// e_{k+1}, the curvature ratios q of j_{k-1} and j_k, and the normals n_k and
+// n_{k+1}, each carried over from the slot before.
 template <class T>
-__device__ __forceinline__ d3 slot_gradient(const MeshView& m, const EParams& p, const Globals& g,
-                                            const double* __restrict__ q, int v, int n, int k, d3 xv, double qv)
+__device__ __forceinline__ d3 ring_gradient(const MeshView& m, const EParams& p, const Globals& g,
+                                            const double* __restrict__ q, int v, int n, d3 xv, double qv)
 {
     const int N = m.nv;
-    const int km = k == 0 ? n - 1 : k - 1, kp = k + 1 == n ? 0 : k + 1;
-    const int jm = m.ring[km * N + v], jc = m.ring[k * N + v], jp = m.ring[kp * N + v];
-    const d3 e_prev = load_p(m.x, jm) - xv;
-    const d3 e_cur = load_p(m.x, jc) - xv;
-    const d3 e_next = load_p(m.x, jp) - xv;
-    const double q_prev = q[jm], q_cur = q[jc];
-    const d3 n_cur = cross_f(e_cur, e_prev);
-    const d3 n_next = cross_f(e_next, e_cur);
-    return slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
+    d3 acc{0, 0, 0};
+    if (n <= kRingPad) {
+        d3 e[kRingPad];
+        double qj[kRingPad];
+#pragma unroll
+        for (int k = 0; k < kRingPad; k++) {
+            const int j = m.ring[k * N + v];
+            const int jj = j >= 0 ? j : v;    // past the valence: v itself, never used
+            e[k] = load_p(m.x, jj) - xv;
+            qj[k] = q[jj];
+        }
+        d3 e_last = e[kRingPad - 1];
+        double q_last = qj[kRingPad - 1];
+#pragma unroll
+        for (int k = kRingPad - 2; k >= 2; k--)
+            if (k == n - 1) {
+                e_last = e[k];
+                q_last = qj[k];
+            }
+        d3 n_cur = cross_f(e[0], e_last);
+#pragma unroll
+        for (int k = 0; k < kRingPad; k++) {
+            if (k < n) {
+                const d3 e_next = k + 1 < n ? e[(k + 1) % kRingPad] : e[0];
+                const d3 e_prev = k == 0 ? e_last : e[(k + kRingPad - 1) % kRingPad];
+                const double q_prev = k == 0 ? q_last : qj[(k + kRingPad - 1) % kRingPad];
+                const d3 n_next = cross_f(e_next, e[k]);
+                acc += slot_gradient_at<T>(p, g, qv, e_prev, e[k], q_prev, qj[k], n_cur, n_next);
+                n_cur = n_next;
+            }
+        }
+        return acc;
+    }
+    const int j0 = m.ring[v], jl = m.ring[(n - 1) * N + v];
+    const d3 e_first = load_p(m.x, j0) - xv;
+    const double q_first = q[j0];
+    d3 e_prev = load_p(m.x, jl) - xv;
+    double q_prev = q[jl];
+    d3 e_cur = e_first;
+    double q_cur = q_first;
+    d3 n_cur = cross_f(e_cur, e_prev);
+    for (int k = 0; k < n; k++) {
+        d3 e_next = e_first;
+        double q_next = q_first;
+        if (k + 1 < n) {
+            const int j = m.ring[(k + 1) * N + v];
+            e_next = load_p(m.x, j) - xv;
+            q_next = q[j];
+        }
+        const d3 n_next = cross_f(e_next, e_cur);
+        acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
+        e_prev = e_cur;
+        q_prev = q_cur;
+        e_cur = e_next;
+        q_cur = q_next;
+        n_cur = n_next;
+    }
+    return acc;
 }
 
 template <class T, int kThreads>
@@ -1413,55 +1363,31 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
     const EParams& p = A.p;
-    const int t = lb * kThreads + threadIdx.x;
-    const int v = t / kGroup, lane = t % kGroup;
+    const int v = lb * kThreads + threadIdx.x;
     const bool active = v < m.nv;
     const int meta = active ? m.meta[v] : 0;
-    const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
     d3 acc{0, 0, 0};
-    const int jl = active ? m.ring[lane * m.nv + v] : -1;    // as in pass 1
     if (ring) {
-        const Globals g = *A.globals;
-        const d3 xv = load_p(m.x, v);
-        const double qv = A.q[v];
-        if (n <= kGroup) {
-            // as in pass 1: e_{k-1}, q_{k-1} from lane k-1, n_{k+1} from lane k+1
-            const RingLanes r = ring_lanes(n, lane);
-            const int jc = jl >= 0 ? jl : v;
-            const d3 e_cur = load_p(m.x, jc) - xv;
-            const double q_cur = A.q[jc];
-            const d3 e_prev = shfl_d3(e_cur, r.src_prev);
-            const double q_prev = __shfl(q_cur, r.src_prev, 64);
-            const d3 n_cur = cross_f(e_cur, e_prev);
-            const d3 n_next = shfl_d3(n_cur, r.src_next);
-            if (lane < n) acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
-        } else {
-            for (int k = lane; k < n; k += kGroup) acc += slot_gradient<T>(m, p, g, A.q, v, n, k, xv, qv);
-        }
+        acc = ring_gradient<T>(m, p, *A.globals, A.q, v, meta_valence(meta), load_p(m.x, v), A.q[v]);
+        // repulsion gradient of the vertices the query found pairs for
+        // (written by the query kernel of this evaluation)
+        if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0)
+            acc += d3{A.rep_g[3 * v], A.rep_g[3 * v + 1], A.rep_g[3 * v + 2]};
     }
-    acc.x = group_sum(acc.x);
-    acc.y = group_sum(acc.y);
-    acc.z = group_sum(acc.z);
-    // repulsion gradient of the vertices the query found pairs for (written
-    // by the query kernel of this evaluation)
-    if (lane == 0 && ring && A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0)
-        acc += d3{A.rep_g[3 * v], A.rep_g[3 * v + 1], A.rep_g[3 * v + 2]};
-    // stage the block's 32 gradient rows and store them as contiguous doubles;
+    // stage the block's gradient rows and store them as contiguous doubles;
     // rows of listed (boundary) vertices are left to k_generic_grad
-    __shared__ double s_g[3 * kThreads / kGroup];
-    __shared__ int s_ring[kThreads / kGroup];
-    if (lane == 0) {
-        const int r = threadIdx.x / kGroup;
-        s_g[3 * r] = acc.x;
-        s_g[3 * r + 1] = acc.y;
-        s_g[3 * r + 2] = acc.z;
-        s_ring[r] = ring;
-    }
+    __shared__ double s_g[3 * kThreads];
+    __shared__ int s_ring[kThreads];
+    s_g[3 * threadIdx.x] = acc.x;
+    s_g[3 * threadIdx.x + 1] = acc.y;
+    s_g[3 * threadIdx.x + 2] = acc.z;
+    s_ring[threadIdx.x] = ring;
     __syncthreads();
-    const int v0 = (lb * kThreads) / kGroup;
-    const int i = threadIdx.x;
-    if (i < 3 * kThreads / kGroup) {
+    const int v0 = lb * kThreads;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int i = c * kThreads + threadIdx.x;
         const int r = i / 3;
         if (v0 + r < m.nv && s_ring[r]) A.grad[3 * v0 + i] = s_g[i];
     }
@@ -1485,7 +1411,7 @@ __global__ void __launch_bounds__(kThreads) k_pass2_grouped(const PassArgs* __re
 // ------------------------------------------------------------ launchers ---
 static inline int nblocks(int n, int b) { return (n + b - 1) / b; }
 
-int pass1_blocks(int nv) { return nblocks(nblocks(nv * kGroup, 256), kXcds) * kXcds; }
+int pass1_blocks(int nv) { return nblocks(nblocks(nv, kPass1Verts), kXcds) * kXcds; }
 
 PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
                    double* grad)
@@ -1520,6 +1446,7 @@ bool tuning_valid(const Tuning& t)
 }
 
 int tuned_blocks(int nv, int threads) { return nblocks(nblocks(nv * kGroup, threads), kXcds) * kXcds; }
+int pass2_blocks(int nv, int threads) { return nblocks(nblocks(nv, threads), kXcds) * kXcds; }
 
 template <int TH, int B>
 static void rep_go(bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
@@ -1555,7 +1482,7 @@ static void p2_go(bool grouped, const PassArgs& a, const GroupLaunch* g, hipStre
     if (grouped)
         hipLaunchKernelGGL((k_pass2_grouped<T, TH>), dim3(g->total_p2), dim3(TH), 0, s, g->d_args, g->d_block0_p2,
                            g->R);
-    else hipLaunchKernelGGL((k_pass2<T, TH>), dim3(tuned_blocks(a.m.nv, TH)), dim3(TH), 0, s, a);
+    else hipLaunchKernelGGL((k_pass2<T, TH>), dim3(pass2_blocks(a.m.nv, TH)), dim3(TH), 0, s, a);
 }
 template <class T>
 static void p2_threads(int th, bool grouped, const PassArgs& a, const GroupLaunch* g, hipStream_t s)
@@ -1574,38 +1501,13 @@ static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs
     else p2_threads<double>(t.p2_threads, grouped, a, g, s);
 }
 
-template <class T>
-static void pass1_rep_go(int batch, const PassArgs& a, hipStream_t s)
-{
-    const dim3 grid(2 * a.nblocks), block(256);
-    switch (batch) {
-    case 1: hipLaunchKernelGGL((k_pass1_rep<T, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_pass1_rep<T, 2>), grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_pass1_rep<T, 8>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_pass1_rep<T, 4>), grid, block, 0, s, a); break;
-    }
-}
-
+// The repulsion query (its own launch: a chain of dependent loads with few
+// registers), then pass 1, which reads the query's rep_v.
 void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
 {
     if (a.m.nv == 0) return;
-    if (a.rep_v && a.pc.x_snap) {
-        // the cached query writes the repulsion column of the partials itself
-        // (a second stream for it, beside pass 1, measured slower: the
-        // cross-queue fork and join cost more than the overlap gains)
-        launch_rep(a, t, s);
-        const dim3 grid(a.nblocks), block(256);
-        if (mixed) hipLaunchKernelGGL(k_pass1_apart<float>, grid, block, 0, s, a);
-        else hipLaunchKernelGGL(k_pass1_apart<double>, grid, block, 0, s, a);
-        return;
-    }
-    if (a.rep_v && t.rep_fused) {
-        if (mixed) pass1_rep_go<float>(t.rep_batch, a, s);
-        else pass1_rep_go<double>(t.rep_batch, a, s);
-        return;
-    }
     launch_rep(a, t, s);
-    dim3 grid(a.nblocks), block(256);
+    const dim3 grid(a.nblocks), block(kPass1Verts);
     if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
     else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
 }
@@ -1690,15 +1592,9 @@ void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
             hipLaunchKernelGGL(k_generic_props_at, dim3(nblocks(g.h_args[r].nlist, 128)), dim3(128), 0, s, g.d_args,
                                r);
     launch_rep_grouped(g, s);
-    dim3 grid(g.total_p1), block(256);
-    if (g.total_rc > 0) {
-        if (g.mixed) hipLaunchKernelGGL(k_pass1_apart_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-        else hipLaunchKernelGGL(k_pass1_apart_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    } else if (g.mixed) {
-        hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    } else {
-        hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    }
+    const dim3 grid(g.total_p1), block(kPass1Verts);
+    if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
     hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(1024), 0, s, g.d_args);
 }
 

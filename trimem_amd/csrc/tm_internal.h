@@ -40,6 +40,7 @@ struct PairCache {
     int force;              // single meshes: 1 records in this launch
 };
 constexpr int kCachedVerts = 256;    // vertices per block of the cached query
+constexpr int kPass1Verts = 256;     // vertices (one lane each) per block of pass 1
 
 // energy passes (tm_energy.hip)
 struct PassBuffers {
@@ -77,7 +78,7 @@ struct PassArgs {
     double* vprops;      // optional per-vertex properties of pass 1
     const int* list;     // boundary / degenerate vertices
     int nlist;
-    int nblocks;         // pass-1 blocks (a multiple of 8)
+    int nblocks;         // pass-1 blocks of kPass1Verts vertices (a multiple of 8)
     PairCache pc;
 };
 PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, const PassBuffers& b, double* vprops,
@@ -97,7 +98,8 @@ struct Tuning {
     int serial_window = 256;  // proposals per window of the serial sweep (1..256), 0: one workgroup
 };
 bool tuning_valid(const Tuning& t);
-int tuned_blocks(int nv, int threads);    // lane-group blocks, a multiple of 8
+int tuned_blocks(int nv, int threads);    // blocks of 8-lane groups (the query), a multiple of 8
+int pass2_blocks(int nv, int threads);    // blocks of one lane per vertex (pass 2), a multiple of 8
 // the repulsion query and the pass-2 kernel alone (for timing)
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s);
 // the repulsion query and the pass-1 kernel (fused or apart), no finalize
