@@ -106,6 +106,19 @@ int tm_halfedge_geometry(tm_mesh* mesh, int he, double* out47);
 /* ---- utilities (util.cpp:9-41, bindings.cpp:987-1036) ---- */
 int tm_area(tm_mesh* mesh, double* out);
 int tm_edges_length(tm_mesh* mesh, double* out);
+/* No counterpart in the reference: tm_area (what = 0) or tm_edges_length
+ * (what = 1) for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * It sums over the device-resident positions (tm_mesh_device_points; the
+ * mesh must already be on the device, e.g. after tm_mesh_upload_points),
+ * keeps its block partials and arrival counter in a fixed scratch array of
+ * the library (one per GPU, held until the process ends; meshes of at most
+ * 8388608 faces or edges, larger ones are refused with TM_E_INVALID), and
+ * writes the sum to *out, a double in device memory.  It calls no hipMalloc
+ * or hipFree and does not synchronise (tm_sync, or the library stream from
+ * tm_stream, orders the result).  All calls share that scratch: calls that
+ * overlap on one GPU (from several threads) race. */
+int tm_mesh_sum_noalloc(tm_mesh* mesh, int what, double* out);
 
 /* ---- energy: EnergyManager (energy.h:18-56, energy.cpp:13-151) ---- */
 /* EnergyManager(mesh, eparams) (energy.cpp:13-28); bond type None and

@@ -57,10 +57,23 @@ struct TmError : std::runtime_error {
     return TM_OK;
 
 // ------------------------------------------------------------ runtime -----
+// Scratch of tm_mesh_sum_noalloc: the arrival counter of its reduction (the
+// first kNoAllocCounterBytes) and one partial sum per block of 256 faces or
+// edges, in memory the library owns for the life of the process (one copy on
+// every GPU it runs on), so that entry point allocates nothing.  Host code
+// never passes this name to a kernel: it stands for a host-side shadow, and
+// the device address comes from hipGetSymbolAddress (g_rt.scratch).
+constexpr int kNoAllocMaxBlocks = 32768;                     // at most 32768 * 256 faces or edges
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr int kNoAllocScratchDoubles = kNoAllocCounterBytes / sizeof(double) + kNoAllocMaxBlocks;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+constexpr int kMaxDevices = 64;
+
 struct Runtime {
     bool init = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    void* scratch[kMaxDevices] = {};    // device address of `scratch` on each device
 };
 Runtime g_rt;
 
@@ -69,8 +82,15 @@ void ensure_runtime()
     if (g_rt.init) return;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw TmError(TM_E_DEVICE, "no HIP device available");
+    if (g_rt.device < 0 || g_rt.device >= n || g_rt.device >= kMaxDevices)
+        throw TmError(TM_E_DEVICE, "device index out of range");
     HIPCHECK(hipSetDevice(g_rt.device));
     HIPCHECK(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+    // the scratch array's address on this device, with the device current
+    void* sp = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    if (!sp) throw TmError(TM_E_DEVICE, "no device address for the library scratch");
+    g_rt.scratch[g_rt.device] = sp;
     g_rt.init = true;
 }
 
@@ -934,6 +954,33 @@ static int sum_geometry(tm_mesh* mesh, int what, double* out)
 
 int tm_area(tm_mesh* mesh, double* out) { return sum_geometry(mesh, 0, out); }
 int tm_edges_length(tm_mesh* mesh, double* out) { return sum_geometry(mesh, 1, out); }
+
+// tm_area / tm_edges_length without allocating: the device-resident
+// positions, the partials and counter in the library's scratch, the sum
+// written to the caller's device double, nothing synchronised.
+int tm_mesh_sum_noalloc(tm_mesh* mesh, int what, double* out)
+{
+    API_BEGIN
+    require(mesh != nullptr && out != nullptr, "null argument");
+    require(what == 0 || what == 1, "what must be 0 (face areas) or 1 (edge lengths)");
+    ensure_runtime();
+    require(mesh->dev, "the mesh is not on the device yet (upload its points first)");
+    const int n = what == 0 ? mesh->nf : mesh->nh / 2;
+    require((n + 255) / 256 <= kNoAllocMaxBlocks, "too many faces or edges for the allocation-free sum (at most 8388608)");
+    char* base = (char*)g_rt.scratch[g_rt.device];
+    unsigned* counter = (unsigned*)base;
+    double* partials = (double*)(base + kNoAllocCounterBytes);
+    if (n > 0) {
+        // the last-block counter starts at zero (a fill kernel, not a memset:
+        // the call may be captured into a graph)
+        launch_fill_ints((int*)counter, 1, 0, nullptr, 0, 0, nullptr, 0, 0, g_rt.stream);
+        launch_sum_geometry(mesh->view(), what, partials, counter, out, g_rt.stream);
+    } else {
+        launch_fill_ints((int*)out, 2, 0, nullptr, 0, 0, nullptr, 0, 0, g_rt.stream);    // +0.0
+    }
+    check_launch();
+    API_END
+}
 
 // ---- energy ----
 int tm_energy_create(tm_mesh* mesh, const tm_params* params, tm_energy** out)
