@@ -354,6 +354,25 @@ int tm_surrogate_train(tm_surrogate* sg, int epochs, float lr, uint64_t seed, do
 /* tm_hmc_step with the surrogate's forces and the exact energies */
 int tm_hmc_step_surrogate(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* params, double temperature,
                           const double* momenta, double uniform, uint64_t seed, uint64_t step);
+/* One HMC step (trimem/mc/hmc.py:133-158) driven by the surrogate while it
+ * earns its keep, by the exact gradient otherwise; every step is accepted
+ * with the exact energy either way.  The acceptance of the last `window`
+ * surrogate-driven steps is kept; when it falls below `min_acceptance` the
+ * exact gradient (tm_hmc_step) drives, except one surrogate-driven probe
+ * every `probe_every` steps, and once `window` probes accept at that rate
+ * again the surrogate drives again.  Topologies the network cannot take
+ * (boundary, valence above 10) are driven by the exact gradient.  driver
+ * (may be NULL): 1 surrogate, 0 exact.  Reading a surrogate-driven step's
+ * outcome makes the next guarded step wait for it.  Defaults: window 10,
+ * min_acceptance 0.5, probe_every 10. */
+int tm_hmc_step_guarded(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* params, double temperature,
+                        const double* momenta, double uniform, uint64_t seed, uint64_t step, int* driver);
+int tm_surrogate_set_policy(tm_surrogate* sg, int window, double min_acceptance, int probe_every);
+/* falling_back: 1 while the exact gradient drives; counts of surrogate- and
+ * exact-driven guarded steps and of switches; the acceptance over the
+ * current window (-1 if empty).  Any pointer may be NULL. */
+int tm_surrogate_policy_state(tm_surrogate* sg, int* falling_back, long long* n_surrogate, long long* n_exact,
+                              long long* n_switches, double* recent_acceptance);
 
 #ifdef __cplusplus
 }

@@ -181,3 +181,42 @@ def test_surrogate_hmc_uses_exact_energies(core):
         else:
             assert np.array_equal(x, x0)
         print(s, r["accepted"], r["dh"], flush=True)
+
+
+def test_guarded_steps_fall_back_to_the_exact_gradient(core):
+    """The plain-gradient fallback: a deliberately bad network (its weights
+    scaled by 50, so its forces blow trajectories apart) drives the first
+    `window` guarded steps, none of which is accepted; the exact gradient
+    then drives, except one surrogate-driven probe every `probe_every`
+    steps.  The exact-driven steps are tm_hmc_step's steps: the positions
+    equal those of a plain device chain given the same (seed, step) draws,
+    and they are accepted at the plain chain's rate."""
+    v, f, mesh, est, om, oe = system(core, 3, seed=17)
+    est.precision = "fp64"
+    sg = core.Surrogate(mesh, est, seed=9)
+    sg.params = np.asarray(sg.params) * 50.0
+    sg.set_policy(window=5, min_acceptance=0.5, probe_every=4)
+    d = core.DeviceHMC(mesh, est)
+    _, _, lbar = M.jittered_icosphere(3, 17)
+    mesh2 = core.TriMesh(v, f)
+    est2 = core.EnergyManager(mesh2, core_params(core, pdict(lbar)))
+    est2.precision = "fp64"
+    plain = core.DeviceHMC(mesh2, est2)
+    kw = dict(mass=1.0, time_step=1e-3, num_integration_steps=6, temperature=1.0, seed=31)
+    drivers, accepted = [], []
+    for s in range(20):
+        drivers.append(d.step(step=s, surrogate=sg, guarded=True, **kw))
+        accepted.append(d.result()["accepted"])
+        if drivers[-1] == "exact":
+            plain.step(step=s, **kw)
+            assert np.array_equal(mesh.points(), mesh2.points()), s
+    print("drivers", drivers, "accepted", accepted, sg.policy_state(), flush=True)
+    assert drivers[:5] == ["surrogate"] * 5
+    assert not any(a for dr, a in zip(drivers, accepted) if dr == "surrogate")
+    # window full and below 0.5 after step 4: exact from step 5, probes at 8, 12, 16
+    assert [i for i, dr in enumerate(drivers) if dr == "surrogate"] == [0, 1, 2, 3, 4, 8, 12, 16]
+    exact_acc = [a for dr, a in zip(drivers, accepted) if dr == "exact"]
+    assert np.mean(exact_acc) >= 0.5
+    st = sg.policy_state()
+    assert st["falling_back"] and st["switches"] == 1
+    assert st["surrogate_steps"] == 8 and st["exact_steps"] == 12

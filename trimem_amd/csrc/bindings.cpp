@@ -361,8 +361,10 @@ struct DeviceHMC {
     DeviceHMC(const DeviceHMC&) = delete;
     ~DeviceHMC() { tm_hmc_destroy(h); }
 
-    void step(double mass, double time_step, int num_integration_steps, double temperature, py::object momenta,
-              double uniform, uint64_t seed, uint64_t step, double time_step_sq, Surrogate* surrogate)
+    // returns the driver of the step: "exact" or "surrogate"
+    std::string step(double mass, double time_step, int num_integration_steps, double temperature,
+                     py::object momenta, double uniform, uint64_t seed, uint64_t step, double time_step_sq,
+                     Surrogate* surrogate, bool guarded)
     {
         tm_hmc_params p{};
         p.mass = mass;
@@ -377,8 +379,17 @@ struct DeviceHMC {
                 throw std::runtime_error("momenta must have shape (n_vertices, 3)");
             mp = a.data();
         }
-        if (surrogate) check(tm_hmc_step_surrogate(h, surrogate->h, &p, temperature, mp, uniform, seed, step));
-        else check(tm_hmc_step(h, &p, temperature, mp, uniform, seed, step));
+        if (surrogate && guarded) {
+            int driver = 0;
+            check(tm_hmc_step_guarded(h, surrogate->h, &p, temperature, mp, uniform, seed, step, &driver));
+            return driver ? "surrogate" : "exact";
+        }
+        if (surrogate) {
+            check(tm_hmc_step_surrogate(h, surrogate->h, &p, temperature, mp, uniform, seed, step));
+            return "surrogate";
+        }
+        check(tm_hmc_step(h, &p, temperature, mp, uniform, seed, step));
+        return "exact";
     }
     py::dict result()
     {
@@ -709,7 +720,7 @@ PYBIND11_MODULE(core, m)
         .def("step", &DeviceHMC::step, py::kw_only(), py::arg("mass"), py::arg("time_step"), py::arg("num_integration_steps"),
             py::arg("temperature"), py::arg("momenta") = py::none(), py::arg("uniform") = -1.0,
             py::arg("seed") = 0, py::arg("step") = 0, py::arg("time_step_sq") = 0.0,
-            py::arg("surrogate") = nullptr)
+            py::arg("surrogate") = nullptr, py::arg("guarded") = false)
         .def("result", &DeviceHMC::result);
 
     py::class_<Surrogate>(m, "Surrogate")
@@ -766,6 +777,26 @@ PYBIND11_MODULE(core, m)
                 return out;
             },
             py::arg("which"))
+        .def(
+            "set_policy",
+            [](Surrogate& s, int window, double min_acceptance, int probe_every) {
+                check(tm_surrogate_set_policy(s.h, window, min_acceptance, probe_every));
+            },
+            py::arg("window") = 10, py::arg("min_acceptance") = 0.5, py::arg("probe_every") = 10)
+        .def("policy_state",
+             [](Surrogate& s) {
+                 int fb = 0;
+                 long long ns = 0, ne = 0, nsw = 0;
+                 double acc = 0;
+                 check(tm_surrogate_policy_state(s.h, &fb, &ns, &ne, &nsw, &acc));
+                 py::dict d;
+                 d["falling_back"] = (bool)fb;
+                 d["surrogate_steps"] = ns;
+                 d["exact_steps"] = ne;
+                 d["switches"] = nsw;
+                 d["recent_acceptance"] = acc;
+                 return d;
+             })
         .def("add_snapshot", [](Surrogate& s) { check(tm_surrogate_add_snapshot(s.h)); })
         .def("num_snapshots",
              [](Surrogate& s) {

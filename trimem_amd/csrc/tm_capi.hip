@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <numeric>
 #include <random>
@@ -2115,6 +2116,17 @@ struct tm_surrogate {
     DevBuf<float> losses;
     std::vector<std::unique_ptr<DevBuf<double>>> snap_x, snap_g;
     int adam_step = 0;
+    // plain-gradient fallback of tm_hmc_step_guarded
+    int pol_window = 10;         // surrogate-driven outcomes judged together
+    double pol_min_acc = 0.5;    // below this acceptance the exact gradient drives
+    int pol_probe = 10;          // while it does: one surrogate-driven probe every pol_probe steps
+    std::deque<int> recent;      // outcomes (1 accepted) of the last surrogate-driven steps
+    bool fallback = false;
+    int since_probe = 0;
+    bool pending = false;        // the last guarded step was surrogate-driven, its outcome unread
+    unsigned checked_topo = ~0u; // topology whose features were last checked
+    bool topo_ok = false;        // ... and whether the network can take it (closed, valences 3..10)
+    long long n_sg = 0, n_exact = 0, n_switch = 0;
 };
 
 namespace {
@@ -2524,6 +2536,102 @@ int tm_hmc_step_surrogate(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* pr
     launch_hmc_restore(n3, M.x.p, hmc->x0.p, st, g_rt.stream);
     check_launch();
     M.device_ahead = true;
+    API_END
+}
+
+int tm_surrogate_set_policy(tm_surrogate* sg, int window, double min_acceptance, int probe_every)
+{
+    API_BEGIN
+    require(sg != nullptr, "null argument");
+    require(window >= 1 && probe_every >= 1, "window and probe_every must be >= 1");
+    require(min_acceptance >= 0.0 && min_acceptance <= 1.0, "min_acceptance must be in [0, 1]");
+    sg->pol_window = window;
+    sg->pol_min_acc = min_acceptance;
+    sg->pol_probe = probe_every;
+    sg->recent.clear();
+    sg->fallback = false;
+    sg->since_probe = 0;
+    API_END
+}
+
+int tm_surrogate_policy_state(tm_surrogate* sg, int* falling_back, long long* n_surrogate, long long* n_exact,
+                              long long* n_switches, double* recent_acceptance)
+{
+    API_BEGIN
+    require(sg != nullptr, "null argument");
+    const tm_surrogate& S = *sg;
+    if (falling_back) *falling_back = S.fallback;
+    if (n_surrogate) *n_surrogate = S.n_sg;
+    if (n_exact) *n_exact = S.n_exact;
+    if (n_switches) *n_switches = S.n_switch;
+    if (recent_acceptance) {
+        double a = 0;
+        for (int o : S.recent) a += o;
+        *recent_acceptance = S.recent.empty() ? -1.0 : a / S.recent.size();
+    }
+    API_END
+}
+
+int tm_hmc_step_guarded(tm_hmc* hmc, tm_surrogate* sg, const tm_hmc_params* prm, double temperature,
+                        const double* momenta, double uniform, uint64_t seed, uint64_t step, int* driver)
+{
+    API_BEGIN
+    require(hmc && sg && prm, "null argument");
+    tm_surrogate& S = *sg;
+    tm_mesh& M = *hmc->mesh;
+    // the outcome of the previous surrogate-driven step (waits for its
+    // Metropolis kernel), then the switch in either direction
+    if (S.pending) {
+        HmcState st;
+        d2h(&st, hmc->state.p, sizeof(HmcState));
+        sync();
+        S.pending = false;
+        S.recent.push_back(st.accepted ? 1 : 0);
+        while ((int)S.recent.size() > S.pol_window) S.recent.pop_front();
+        if ((int)S.recent.size() == S.pol_window) {
+            double a = 0;
+            for (int o : S.recent) a += o;
+            a /= S.pol_window;
+            if (!S.fallback && a < S.pol_min_acc) {
+                S.fallback = true;
+                S.since_probe = 0;
+                S.recent.clear();
+                S.n_switch++;
+            } else if (S.fallback && a >= S.pol_min_acc) {
+                S.fallback = false;
+                S.n_switch++;
+            }
+        }
+    }
+    bool use_sg = !S.fallback;
+    if (S.fallback && ++S.since_probe >= S.pol_probe) {
+        use_sg = true;
+        S.since_probe = 0;
+    }
+    // the network takes closed meshes with valences 3..10 only; the topology
+    // changes by flips between steps, so it is checked once per topology
+    if (use_sg && S.checked_topo != M.topo_version) {
+        if (!M.device_ahead) push_points(M);
+        sg_feature_pass(S);
+        int e = 0;
+        d2h(&e, S.err.p, sizeof(int));
+        sync();
+        if (e) HIPCHECK(hipMemsetAsync(S.err.p, 0, sizeof(int), g_rt.stream));
+        S.topo_ok = e == 0;
+        S.checked_topo = M.topo_version;
+    }
+    use_sg = use_sg && S.topo_ok;
+    int rc;
+    if (use_sg) {
+        rc = tm_hmc_step_surrogate(hmc, sg, prm, temperature, momenta, uniform, seed, step);
+        S.pending = rc == TM_OK;
+        S.n_sg++;
+    } else {
+        rc = tm_hmc_step(hmc, prm, temperature, momenta, uniform, seed, step);
+        S.n_exact++;
+    }
+    if (rc != TM_OK) throw TmError(rc, tm_last_error());
+    if (driver) *driver = use_sg ? 1 : 0;
     API_END
 }
 
