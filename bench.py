@@ -413,7 +413,7 @@ def _find(d, suffix):
 def _short(name):
     if "copyBuffer" in name:
         return "copy"
-    for k in ("k_disp", "k_rep_cached", "k_rep_record", "k_rep", "k_pass1", "k_pass2",
+    for k in ("k_rep_cached", "k_rep_record", "k_rep", "k_pass1_apart", "k_pass1", "k_pass2",
               "k_finalize", "k_generic_props", "k_generic_grad"):
         if k in name:
             return k + ("_fp64" if "<double>" in name else "")
@@ -421,7 +421,7 @@ def _short(name):
 
 
 # the launches of pass 1 with the pair cache replaying (the default)
-PASS1_KERNELS = ("k_disp", "k_rep_cached", "k_pass1", "k_finalize")
+PASS1_KERNELS = ("k_pass1_apart", "k_rep_cached", "k_finalize")
 SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
                "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU")
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4    # 1024 SIMDs, one wave64 VALU op per 4 cycles at 2.4 GHz

@@ -181,7 +181,9 @@ SHAPES = [dict(rep_threads=64, rep_batch=1, pass2_threads=64, rep_fused=0, pair_
           dict(rep_threads=128, rep_batch=2, pass2_threads=128, rep_fused=0, pair_cache=0, pflip_window=64,
                serial_window=7),
           dict(rep_threads=256, rep_batch=1, pass2_threads=256, rep_fused=1, pair_cache=1, pflip_window=256,
-               serial_window=128)]
+               serial_window=128),
+          dict(rep_threads=256, rep_batch=4, pass2_threads=64, rep_fused=0, pair_cache=1, pflip_window=256,
+               serial_window=256)]
 
 
 def vertex_properties(mesh, est):

@@ -562,9 +562,10 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
         L.pc_cell.alloc(n);
         L.pc_cnt.alloc(n);
         L.pc_ent.alloc((size_t)kPairLanes * kPairCap * n);
-        L.pc_disp.alloc(kDispBlocks);
+        const int nd = pass1_blocks(n);    // one displacement partial per pass-1 block
+        L.pc_disp.alloc(nd);
         // cell -1: no replay; displacement bytes 0x7F (~1e306): a recording first
-        launch_fill_ints(L.pc_cell.p, n, -1, reinterpret_cast<int*>(L.pc_disp.p), 2 * kDispBlocks, 0x7F7F7F7F,
+        launch_fill_ints(L.pc_cell.p, n, -1, reinterpret_cast<int*>(L.pc_disp.p), 2 * nd, 0x7F7F7F7F,
                          nullptr, 0, 0, g_rt.stream);
         check_launch();
         L.pc_ok = false;
@@ -715,7 +716,7 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
         b.pc = pair_cache_of(E, M);
         if (b.pc.x_snap) {
             b.pc.disp_part = L.pc_disp.p;
-            b.pc.disp_n = kDispBlocks;
+            b.pc.disp_n = pass1_blocks(M.nv);
             b.pc.force = L.pc_ok ? 0 : 1;
         }
     }
@@ -1510,15 +1511,15 @@ struct tm_replicas {
     // pair cache: per-replica recording flags (device; the reduction clears
     // them), displacement partials, and the prefix tables of the
     // displacement check and of the cached query
-    DevBuf<int> force_flags, d_block0_disp, d_block0_rc;
+    DevBuf<int> force_flags, d_block0_rc;
     DevBuf<double> disp_parts;
     std::vector<int> h_ones;
-    int total_disp = 0, total_rc = 0;
+    int total_disp = 0, total_rc = 0;    // displacement partials, cached-query blocks
     std::vector<unsigned> gens;    // each member's tm_energy::gen when the table was built
     GroupLaunch group() const
     {
         return GroupLaunch{d_args.p,  h_args.data(), R, d_block0.p, total_blocks, d_block0_rep.p, total_rep,
-                           d_block0_p2.p, total_p2, tune, mixed, d_block0_disp.p, total_disp, d_block0_rc.p,
+                           d_block0_p2.p, total_p2, tune, mixed, d_block0_rc.p,
                            tune.pair_cache ? total_rc : 0};
     }
     // every replica records at its next evaluation
@@ -1572,16 +1573,15 @@ void refresh_replica_args(tm_replicas& S)
     }
     S.h_args.resize(S.R);
     S.h_block0.assign(S.R + 1, 0);
-    // displacement partials: one per 4096 vertices (at most 64) per replica;
-    // cached query: pass1_blocks / 8 blocks of 256 vertices per replica
+    // displacement partials and blocks of the cached query: one per pass-1
+    // block (256 vertices) of each replica
     std::vector<int> b0d(S.R + 1, 0), b0c(S.R + 1, 0);
     for (int r = 0; r < S.R; r++) {
-        b0d[r + 1] = b0d[r] + std::min(std::max((S.meshes[r]->nv + 4095) / 4096, 1), 64);
-        b0c[r + 1] = b0c[r] + (S.meshes[r]->nv + kCachedVerts - 1) / kCachedVerts;
+        b0d[r + 1] = b0d[r] + pass1_blocks(S.meshes[r]->nv);
+        b0c[r + 1] = b0c[r] + pass1_blocks(S.meshes[r]->nv);
     }
     S.total_disp = b0d[S.R];
     S.total_rc = (b0c[S.R] + 7) / 8 * 8;    // a multiple of the 8 XCDs; padding blocks fall past the last replica's vertices
-    S.d_block0_disp.alloc(S.R + 1);
     S.d_block0_rc.alloc(S.R + 1);
     S.disp_parts.alloc(S.total_disp);
     S.force_flags.alloc(S.R);
@@ -1611,7 +1611,6 @@ void refresh_replica_args(tm_replicas& S)
     S.d_block0.alloc(S.R + 1);
     h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
     h2d(S.d_block0.p, S.h_block0.data(), sizeof(int) * (S.R + 1));
-    h2d(S.d_block0_disp.p, b0d.data(), sizeof(int) * (S.R + 1));
     h2d(S.d_block0_rc.p, b0c.data(), sizeof(int) * (S.R + 1));
     set_group_shape(S, S.tune);    // raises every recording flag
 }

@@ -34,24 +34,23 @@
 
 namespace tmh {
 
-// FMA-contracted vector arithmetic for the ring passes' slot terms, whose
-// parity is a tolerance.  The neighbour-list distance tests do not use these:
-// their rounding decides which pairs exist and stays uncontracted, like the
-// reference's.
+// Vector arithmetic of the ring passes' slot terms with explicit fused
+// multiply-adds.  The library is built without FP contraction, so these are
+// the only fusions: the bits depend on the source alone, not on the kernel a
+// function is inlined into (a single-mesh launch and a replica group give the
+// same bits).  The neighbour-list distance tests do not use these: their
+// rounding decides which pairs exist and stays unfused, like the reference's.
 __device__ __forceinline__ d3 cross_f(const d3& a, const d3& b)
 {
-#pragma clang fp contract(fast)
-    return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return d3{fma(a.y, b.z, -(a.z * b.y)), fma(a.z, b.x, -(a.x * b.z)), fma(a.x, b.y, -(a.y * b.x))};
 }
 __device__ __forceinline__ double dot_f(const d3& a, const d3& b)
 {
-#pragma clang fp contract(fast)
-    return a.x * b.x + a.y * b.y + a.z * b.z;
+    return fma(a.x, b.x, fma(a.y, b.y, a.z * b.z));
 }
 __device__ __forceinline__ d3 axpy_f(double s, const d3& a, const d3& y)    // s a + y
 {
-#pragma clang fp contract(fast)
-    return d3{s * a.x + y.x, s * a.y + y.y, s * a.z + y.z};
+    return d3{fma(s, a.x, y.x), fma(s, a.y, y.y), fma(s, a.z, y.z)};
 }
 
 // Scalar kernels of the two precision modes.  Geometry vectors (edge vectors,
@@ -131,6 +130,24 @@ __device__ __forceinline__ int logical_block()
 }
 
 // ------------------------------------------------------------ reduction ---
+// Column k of block_sum6 alone, by the same operations in the same order (a
+// butterfly within each wave, then the waves in order), so its bits are those
+// block_sum6 gives that column; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum1(double a, double* s_red /* [nwaves] */)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) s_red[wave] = a;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < nw; w++) r += s_red[w];
+    return r;
+}
+
 // Sum six per-thread doubles over the block; result valid in s_out[0..5].
 __device__ __forceinline__ void block_sum6(double v[6], double* s_red /* [nwaves*6] */, double* s_out)
 {
@@ -157,52 +174,57 @@ __device__ __forceinline__ void block_sum6(double v[6], double* s_red /* [nwaves
     __syncthreads();
 }
 
-// Fixed-order sum of all block partials, energy and factors.  The partials
-// are stored column by column (partials[k * nblocks + b], property k of
-// block b) so that each load of a wave reads one contiguous run.
-__device__ void finalize_globals(const double* __restrict__ partials, int nblocks, const EParams& p,
-                                 Globals* g, double* s_red, double* s_out)
+// The six globals from the block partials, then the energy (kernel.h:110-122)
+// and the penalty factors 2 kappa / ref (X / ref - 1) (kernel.h:27-71).  One
+// wave computes them: lane i sums rows i, i + 64, ... of each column in that
+// order (loading a sweep of rows before adding it, so their round trips
+// overlap), then a butterfly joins the 64 lanes.  The order depends on
+// nothing but the number of rows, so every kernel that calls this gets the
+// same bits; every lane of the wave returns the result.  The partials are
+// stored column by column (partials[k * nblocks + b]).
+__device__ __forceinline__ Globals reduce_globals(const double* __restrict__ partials, int nblocks, const EParams& p)
 {
-    // Each thread sums blocks tid, tid + blockDim, ... in that order; the rows
-    // of one sweep are all loaded before the first add, so their round trips
-    // overlap instead of following one another.
     constexpr int kRows = 8;
+    const int lane = threadIdx.x & 63;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int b0 = threadIdx.x; b0 < nblocks; b0 += kRows * blockDim.x) {
+    for (int b0 = lane; b0 < nblocks; b0 += kRows * 64) {
         double r[kRows][6];
 #pragma unroll
         for (int u = 0; u < kRows; u++) {
-            const int b = b0 + u * blockDim.x;
+            const int b = b0 + u * 64;
 #pragma unroll
             for (int k = 0; k < 6; k++) r[u][k] = b < nblocks ? partials[k * nblocks + b] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kRows; u++)
-            if (b0 + u * (int)blockDim.x < nblocks) {
+            if (b0 + u * 64 < nblocks) {
 #pragma unroll
                 for (int k = 0; k < 6; k++) v[k] += r[u][k];
             }
     }
-    block_sum6(v, s_red, s_out);
-    if (threadIdx.x == 0) {
-        double A = s_out[0], V = s_out[1], C = s_out[2], B = s_out[3], T = s_out[4], R = s_out[5];
-        for (int k = 0; k < 6; k++) g->props[k] = s_out[k];
-        double e = 0.0, d;
-        d = A / p.ref_area - 1.0;
-        e += p.kappa_a * d * d;
-        g->fac_a = 2.0 * p.kappa_a / p.ref_area * d;
-        d = V / p.ref_volume - 1.0;
-        e += p.kappa_v * d * d;
-        g->fac_v = 2.0 * p.kappa_v / p.ref_volume * d;
-        g->fac_v6 = g->fac_v / 6.0;
-        d = C / p.ref_curvature - 1.0;
-        e += p.kappa_c * d * d;
-        g->fac_c = 2.0 * p.kappa_c / p.ref_curvature * d;
-        e += p.kappa_t * T;
-        e += p.kappa_r * R;
-        e += p.kappa_b * B;
-        g->energy = e;
-    }
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    Globals g;
+    const double A = v[0], V = v[1], C = v[2], B = v[3], T = v[4], R = v[5];
+    for (int k = 0; k < 6; k++) g.props[k] = v[k];
+    double e = 0.0, d;
+    d = A / p.ref_area - 1.0;
+    e += p.kappa_a * d * d;
+    g.fac_a = 2.0 * p.kappa_a / p.ref_area * d;
+    d = V / p.ref_volume - 1.0;
+    e += p.kappa_v * d * d;
+    g.fac_v = 2.0 * p.kappa_v / p.ref_volume * d;
+    g.fac_v6 = g.fac_v / 6.0;
+    d = C / p.ref_curvature - 1.0;
+    e += p.kappa_c * d * d;
+    g.fac_c = 2.0 * p.kappa_c / p.ref_curvature * d;
+    e += p.kappa_t * T;
+    e += p.kappa_r * R;
+    e += p.kappa_b * B;
+    g.energy = e;
+    return g;
 }
 
 // ------------------------------------------------------- lane groups -----
@@ -481,8 +503,21 @@ __device__ __forceinline__ int pair_cache_mode(const PairCache& pc)
     if (pc.force || (pc.dforce && *pc.dforce)) return 2;
     __shared__ int s_mode;
     if (threadIdx.x < 64) {
+        // one partial per pass-1 block: each lane has a batch of loads in
+        // flight before it takes their maximum, instead of one round trip
+        // per partial
+        constexpr int kBatchLoads = 16;
         double mx = 0.0;
-        for (int i = threadIdx.x; i < pc.disp_n; i += 64) mx = fmax(mx, pc.disp_part[i]);
+        for (int i0 = threadIdx.x; i0 < pc.disp_n; i0 += 64 * kBatchLoads) {
+            double d[kBatchLoads];
+#pragma unroll
+            for (int u = 0; u < kBatchLoads; u++) {
+                const int i = i0 + 64 * u;
+                d[u] = i < pc.disp_n ? pc.disp_part[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatchLoads; u++) mx = fmax(mx, d[u]);
+        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
         if (threadIdx.x == 0) s_mode = mx < pc.limit2 ? 1 : 2;
@@ -544,34 +579,6 @@ __device__ __forceinline__ bool cached_neighbours(const MeshView& m, const NList
         if (sq <= dmax2 && sqrt(sq) < dmax) f(j[i], d);
     }
     return true;
-}
-
-// max |x - x_snap|^2 over the vertices, one partial per block (NaN counts as
-// infinite, which forces a recording)
-__device__ __forceinline__ void disp_body(const double* __restrict__ x, const double* __restrict__ xs, int n,
-                                          double* __restrict__ part, int nblk, int blk)
-{
-    __shared__ double s_red[4];
-    double mx = 0.0;
-    for (int v = blk * blockDim.x + threadIdx.x; v < n; v += nblk * blockDim.x) {
-        const double dx = x[3 * v] - xs[3 * v];
-        const double dy = x[3 * v + 1] - xs[3 * v + 1];
-        const double dz = x[3 * v + 2] - xs[3 * v + 2];
-        double q = dx * dx + dy * dy + dz * dz;
-        if (!(q == q)) q = INFINITY;
-        mx = fmax(mx, q);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blk] = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
-}
-
-__global__ void __launch_bounds__(256) k_disp(const double* __restrict__ x, const double* __restrict__ xs, int n,
-                                              double* __restrict__ part)
-{
-    disp_body(x, xs, n, part, gridDim.x, blockIdx.x);
 }
 
 // Exclusion masks of the stored stencil lists: bit i of vertex v is set if
@@ -647,18 +654,22 @@ __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e
 // kRingPad hold -1 past the valence, so a ring of at most kRingPad slots is
 // loaded in full before its valence is known, and its position loads issue
 // together.  Longer rings walk the table slot by slot.
+// use: v is an interior vertex (its sums are wanted).  The leading kRingPad
+// columns and their positions are loaded whether or not they are used, so
+// they need not wait for the meta word that decides it.
 template <class T>
-__device__ __forceinline__ void ring_properties(const MeshView& m, const EParams& p, int v, int n, d3 xv,
+__device__ __forceinline__ void ring_properties(const MeshView& m, const EParams& p, int v, bool use, int n, d3 xv,
                                                 double& area, double& vol, double& curv, double& teth)
 {
     const int N = m.nv;
-    if (n <= kRingPad) {
-        d3 e[kRingPad];
+    d3 e[kRingPad];
 #pragma unroll
-        for (int k = 0; k < kRingPad; k++) {
-            const int j = m.ring[k * N + v];
-            e[k] = load_p(m.x, j >= 0 ? j : v) - xv;    // past the valence: x_v, never used
-        }
+    for (int k = 0; k < kRingPad; k++) {
+        const int j = m.ring[k * N + v];
+        e[k] = load_p(m.x, j >= 0 ? j : v) - xv;    // past the valence: x_v, never used
+    }
+    if (!use) return;
+    if (n <= kRingPad) {
         d3 e_last = e[kRingPad - 1];
 #pragma unroll
         for (int k = kRingPad - 2; k >= 2; k--)
@@ -706,11 +717,18 @@ __device__ __forceinline__ int find_replica(const int* __restrict__ block0, int 
 
 // Pass 1 over the kPass1Verts vertices of block lb; vertices on the boundary
 // (or with fewer than three neighbours) read the properties that
-// k_generic_props wrote.  The repulsion property comes from the query kernel
-// that ran before (rep_v).  Writes q_v, optionally the per-vertex properties,
-// and the block's partial sums of the six globals (FP64, a fixed butterfly
-// order, so the result does not depend on timing).
-template <class T>
+// k_generic_props wrote.  Writes q_v, optionally the per-vertex properties,
+// and the block's partial sums of the six globals (FP64, block_sum6's fixed
+// butterfly order, so the result does not depend on timing).
+//
+// The repulsion property (column 5) comes from the query kernel.  Without
+// the pair cache that kernel runs first and this one reads rep_v.  With it
+// (kApart) the cached query runs after this kernel and sums column 5 of
+// the same 256-vertex blocks itself, in the same order; this kernel then
+// measures how far every vertex has moved since the cache's recording (the
+// largest squared displacement of the block, NaN counting as infinite), the
+// query's replay condition.
+template <class T, bool kApart = false>
 __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
 {
     const MeshView& m = A.m;
@@ -719,14 +737,23 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     __shared__ double s_out[6];
     const int v = lb * kPass1Verts + threadIdx.x;
     const bool active = v < m.nv;
-    const int meta = active ? m.meta[v] : 0;
-    const double rv = (A.rep_v && active) ? A.rep_v[v] : 0.0;
+    const int vc = active ? v : m.nv - 1;    // lanes past the last vertex load a valid one
+    const int meta = m.meta[vc];
+    const double rv = (!kApart && A.rep_v && active) ? A.rep_v[v] : 0.0;
+    double disp = 0.0;
+    if (kApart && A.pc.x_snap && active) {
+        const double* xs = A.pc.x_snap;
+        const double dx = m.x[3 * v] - xs[3 * v], dy = m.x[3 * v + 1] - xs[3 * v + 1];
+        const double dz = m.x[3 * v + 2] - xs[3 * v + 2];
+        disp = dx * dx + dy * dy + dz * dz;
+        if (!(disp == disp)) disp = INFINITY;
+    }
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);
     double pr[6] = {0, 0, 0, 0, 0, 0};
+    double area = 0, vol = 0, curv = 0, teth = 0;
+    ring_properties<T>(m, p, vc, ring, n, load_p(m.x, vc), area, vol, curv, teth);
     if (ring) {
-        double area = 0, vol = 0, curv = 0, teth = 0;
-        ring_properties<T>(m, p, v, n, load_p(m.x, v), area, vol, curv, teth);
         // a third of the face areas and volumes, half the edge terms (by
         // the rounded reciprocals: at most the last bit differs from a
         // division, far inside the parity tolerance)
@@ -744,15 +771,31 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
         A.q[v] = pr[2] / pr[0];
     }
     if (A.vprops && active)
-        for (int k = 0; k < 6; k++) A.vprops[6 * v + k] = pr[k];
+        for (int k = 0; k < (kApart ? 5 : 6); k++) A.vprops[6 * v + k] = pr[k];
+    if (kApart && A.pc.x_snap) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) disp = fmax(disp, __shfl_xor(disp, off, 64));
+        __shared__ double s_disp[kPass1Verts / 64];
+        if ((threadIdx.x & 63) == 0) s_disp[threadIdx.x >> 6] = disp;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            A.pc.disp_part[lb] = fmax(fmax(s_disp[0], s_disp[1]), fmax(s_disp[2], s_disp[3]));
+    }
     block_sum6(pr, s_red, s_out);
-    if (threadIdx.x < 6) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
+    if (threadIdx.x < (kApart ? 5 : 6)) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
 }
 
 template <class T>
 __global__ void __launch_bounds__(kPass1Verts) k_pass1(PassArgs A)
 {
     pass1_body<T>(A, logical_block());
+}
+
+// pass 1 before the cached repulsion query (column 5 left to it)
+template <class T>
+__global__ void __launch_bounds__(kPass1Verts) k_pass1_apart(PassArgs A)
+{
+    pass1_body<T, true>(A, logical_block());
 }
 
 template <class T>
@@ -762,6 +805,18 @@ __global__ void __launch_bounds__(kPass1Verts) k_pass1_grouped(const PassArgs* _
     const int lb = logical_block();
     const int r = find_replica(block0, R, lb);
     pass1_body<T>(A[r], lb - block0[r]);
+}
+
+// replica groups with the cached query: every replica with repulsion leaves
+// column 5 to it
+template <class T>
+__global__ void __launch_bounds__(kPass1Verts) k_pass1_apart_grouped(const PassArgs* __restrict__ A,
+                                                                     const int* __restrict__ block0, int R)
+{
+    const int lb = logical_block();
+    const int r = find_replica(block0, R, lb);
+    if (A[r].rep_v) pass1_body<T, true>(A[r], lb - block0[r]);
+    else pass1_body<T, false>(A[r], lb - block0[r]);
 }
 
 // Repulsion property of every ring vertex (mesh_repulsion.h:28-54):
@@ -977,6 +1032,7 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
     const unsigned rec_word = mine && pc.x_snap ? pc.cnt[v] : 0u;
     const int mode = pc.x_snap ? pair_cache_mode(pc) : 0;    // block-uniform: 1 replay, 2 record, 0 full
     __shared__ int s_list[kVertsPerBlock];
+    __shared__ double s_val[kVertsPerBlock];    // repulsion property per vertex (column 5 of pass 1)
     __shared__ int s_n;
     __shared__ int s_rest[kVertsPerBlock];
     __shared__ int s_nrest;
@@ -985,6 +1041,7 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         s_nrest = 0;
     }
     __syncthreads();
+    if (threadIdx.x < kVertsPerBlock) s_val[threadIdx.x] = 0.0;
     if (mine) {
         const bool ring = !is_generic(meta);
         bool done = !ring;
@@ -1036,6 +1093,7 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         if (done) {
             A.rep_v[v] = rep;
             A.pair_cnt[v] = cnt;
+            s_val[threadIdx.x] = ring ? rep : A.gprops[6 * v + 5];
         } else {
             s_list[atomicAdd(&s_n, 1)] = v;
         }
@@ -1057,6 +1115,7 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
                 if ((threadIdx.x & 63) == 0) {
                     A.rep_v[u] = tr.rep;
                     A.pair_cnt[u] = tr.cnt;
+                    s_val[u - base] = tr.rep;
                     store_rep_g(A, u, tr);
                 }
             } else if ((threadIdx.x & 63) == 0) {
@@ -1083,9 +1142,19 @@ __device__ __forceinline__ void rep_cached_body(const PassArgs& A, int lb)
         if (lane == 0) {
             A.rep_v[u] = tr.rep;
             A.pair_cnt[u] = tr.cnt;
+            s_val[u - base] = tr.rep;
             store_rep_g(A, u, tr);
         }
     }
+    __syncthreads();
+    // column 5 of pass-1 block lb (the same 256 vertices), summed by the
+    // function and in the order pass 1 sums its columns
+    static_assert(kVertsPerBlock == kPass1Verts, "the cached query covers one pass-1 block");
+    const int vi = base + threadIdx.x;
+    if (A.vprops && vi < m.nv) A.vprops[6 * vi + 5] = s_val[threadIdx.x];
+    __shared__ double s_red[kPass1Verts / 64];
+    const double col5 = block_sum1(s_val[threadIdx.x], s_red);
+    if (threadIdx.x == 0 && lb < A.nblocks) A.partials[5 * A.nblocks + lb] = col5;
 }
 
 template <int kBatch>
@@ -1103,15 +1172,6 @@ __global__ void __launch_bounds__(256) k_rep_cached_grouped(const PassArgs* __re
     const int lb = logical_block();
     const int r = find_replica(block0, R, lb);
     rep_cached_body<kBatch>(A[r], lb - block0[r]);
-}
-
-__global__ void __launch_bounds__(256) k_disp_grouped(const PassArgs* __restrict__ A, const int* __restrict__ block0,
-                                                      int R)
-{
-    const int r = find_replica(block0, R, blockIdx.x);
-    const PassArgs& a = A[r];
-    if (!a.rep_v || !a.pc.x_snap) return;
-    disp_body(a.m.x, a.pc.x_snap, a.m.nv, a.pc.disp_part, block0[r + 1] - block0[r], blockIdx.x - block0[r]);
 }
 
 // A recording launch the host asks for (new list or topology): the layout
@@ -1170,22 +1230,22 @@ __global__ void __launch_bounds__(kThreads) k_rep_grouped(const PassArgs* __rest
 }
 
 // Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
-// factors (one block; kernel boundaries order it after pass 1).
-__global__ void __launch_bounds__(1024) k_finalize(PassArgs A)
+// factors (one wave; kernel boundaries order it after pass 1).
+__global__ void __launch_bounds__(64) k_finalize(PassArgs A)
 {
-    __shared__ double s_red[16 * 6];
-    __shared__ double s_out[6];
-    finalize_globals(A.partials, A.nblocks, A.p, A.globals, s_red, s_out);
+    const Globals g = reduce_globals(A.partials, A.nblocks, A.p);
+    if (threadIdx.x == 0) *A.globals = g;
 }
 
-// one block per replica
-__global__ void __launch_bounds__(1024) k_finalize_grouped(const PassArgs* __restrict__ A)
+// one wave per replica
+__global__ void __launch_bounds__(64) k_finalize_grouped(const PassArgs* __restrict__ A)
 {
-    __shared__ double s_red[16 * 6];
-    __shared__ double s_out[6];
     const PassArgs& a = A[blockIdx.x];
-    finalize_globals(a.partials, a.nblocks, a.p, a.globals, s_red, s_out);
-    if (threadIdx.x == 0 && a.pc.dforce) *a.pc.dforce = 0;    // the query of this evaluation recorded
+    const Globals g = reduce_globals(a.partials, a.nblocks, a.p);
+    if (threadIdx.x == 0) {
+        *a.globals = g;
+        if (a.pc.dforce) *a.pc.dforce = 0;    // the query of this evaluation recorded
+    }
 }
 
 // OpenMesh-semantics properties of the listed vertices (boundary and
@@ -1263,7 +1323,6 @@ template <class T>
 __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& g, double qv, d3 e_prev, d3 e_cur,
                                                double q_prev, double q_cur, d3 n_cur, d3 n_next)
 {
-#pragma clang fp contract(fast)
     const double kb = p.kappa_b, kt = p.kappa_t;
     // face F_k = (v, j_k, j_{k-1})
     T n0l = Prec<T>::sqrt_(dot_f(n_cur, n_cur));
@@ -1273,10 +1332,10 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     T cot_i = (T)dot_f(e_prev, rim) * inv0;                // angle at j_{k-1}
     d3 u = (double)inv0 * n_cur;
     d3 dA = 0.5 * cross_f(u, rim);                         // d area(F_k) / d x_v
-    double wq = (double)cot_j * (qv - q_prev) + (double)cot_i * (qv - q_cur);
-    double q2 = qv * qv + q_cur * q_cur + q_prev * q_prev;
-    double fdA = g.fac_a - kb * (2.0 / 3.0) * q2;
-    if (p.bond_type == 1) fdA += kt * bond_area_dfac((double)(n0l * (T)0.5), p);
+    double wq = fma((double)cot_j, qv - q_prev, (double)cot_i * (qv - q_cur));
+    double q2 = fma(qv, qv, fma(q_cur, q_cur, q_prev * q_prev));
+    double fdA = fma(-kb * (2.0 / 3.0), q2, g.fac_a);
+    if (p.bond_type == 1) fdA = fma(kt, bond_area_dfac((double)(n0l * (T)0.5), p), fdA);
     d3 acc = fdA * dA;
     acc = axpy_f(g.fac_v6, n_cur, acc);
     acc = axpy_f(kb * wq, u, acc);
@@ -1284,53 +1343,69 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     const double l2 = dot_f(e_cur, e_cur);
     T l = Prec<T>::sqrt_(l2);
     T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
-    double fe = -(kb * (qv + q_cur) + 0.5 * g.fac_c) * (double)th;
-    if (p.bond_type == 0) fe -= kt * tether_dfac<T>(l2, l, p);
+    double fe = -fma(kb, qv + q_cur, 0.5 * g.fac_c) * (double)th;
+    if (p.bond_type == 0) fe = fma(-kt, tether_dfac<T>(l2, l, p), fe);
     return axpy_f(fe * (double)Prec<T>::rcp(l), e_cur, acc);
 }
 
 // The gradient at interior vertex v, one lane per vertex, walking the ring
 // as pass 1 does (ring_properties): slot k needs the spokes e_{k-1}, e_k,
 // e_{k+1}, the curvature ratios q of j_{k-1} and j_k, and the normals n_k and
-// n_{k+1}, each carried over from the slot before.
+// n_{k+1}, each carried over from the slot before.  A ring of at most
+// kRingPad slots is loaded in full first.
+// The gradient of a ring of at most kRingPad neighbours.  Every lane loads
+// the leading kRingPad ring columns of vc (they hold -1 past the valence,
+// which loads vc itself) and their positions and q before it knows whether
+// it needs them (use: an interior vertex with such a ring), so the loads do
+// not wait for the vertex's meta word.
 template <class T>
-__device__ __forceinline__ d3 ring_gradient(const MeshView& m, const EParams& p, const Globals& g,
-                                            const double* __restrict__ q, int v, int n, d3 xv, double qv)
+__device__ __forceinline__ d3 ring_gradient_short(const MeshView& m, const EParams& p, const Globals* __restrict__ gp,
+                                                  const double* __restrict__ q, int vc, bool use, int n, d3 xv,
+                                                  double qv)
+{
+    const int N = m.nv;
+    d3 e[kRingPad];
+    double qj[kRingPad];
+#pragma unroll
+    for (int k = 0; k < kRingPad; k++) {
+        const int j = m.ring[k * N + vc];
+        const int jj = j >= 0 ? j : vc;
+        e[k] = load_p(m.x, jj) - xv;
+        qj[k] = q[jj];
+    }
+    d3 acc{0, 0, 0};
+    if (!use) return acc;
+    const Globals g = *gp;
+    d3 e_last = e[kRingPad - 1];
+    double q_last = qj[kRingPad - 1];
+#pragma unroll
+    for (int k = kRingPad - 2; k >= 2; k--)
+        if (k == n - 1) {
+            e_last = e[k];
+            q_last = qj[k];
+        }
+    d3 n_cur = cross_f(e[0], e_last);
+#pragma unroll
+    for (int k = 0; k < kRingPad; k++) {
+        if (k < n) {
+            const d3 e_next = k + 1 < n ? e[(k + 1) % kRingPad] : e[0];
+            const d3 e_prev = k == 0 ? e_last : e[(k + kRingPad - 1) % kRingPad];
+            const double q_prev = k == 0 ? q_last : qj[(k + kRingPad - 1) % kRingPad];
+            const d3 n_next = cross_f(e_next, e[k]);
+            acc += slot_gradient_at<T>(p, g, qv, e_prev, e[k], q_prev, qj[k], n_cur, n_next);
+            n_cur = n_next;
+        }
+    }
+    return acc;
+}
+
+// rings longer than kRingPad, slot by slot
+template <class T>
+__device__ __forceinline__ d3 ring_gradient_walk(const MeshView& m, const EParams& p, const Globals& g,
+                                                 const double* __restrict__ q, int v, int n, d3 xv, double qv)
 {
     const int N = m.nv;
     d3 acc{0, 0, 0};
-    if (n <= kRingPad) {
-        d3 e[kRingPad];
-        double qj[kRingPad];
-#pragma unroll
-        for (int k = 0; k < kRingPad; k++) {
-            const int j = m.ring[k * N + v];
-            const int jj = j >= 0 ? j : v;    // past the valence: v itself, never used
-            e[k] = load_p(m.x, jj) - xv;
-            qj[k] = q[jj];
-        }
-        d3 e_last = e[kRingPad - 1];
-        double q_last = qj[kRingPad - 1];
-#pragma unroll
-        for (int k = kRingPad - 2; k >= 2; k--)
-            if (k == n - 1) {
-                e_last = e[k];
-                q_last = qj[k];
-            }
-        d3 n_cur = cross_f(e[0], e_last);
-#pragma unroll
-        for (int k = 0; k < kRingPad; k++) {
-            if (k < n) {
-                const d3 e_next = k + 1 < n ? e[(k + 1) % kRingPad] : e[0];
-                const d3 e_prev = k == 0 ? e_last : e[(k + kRingPad - 1) % kRingPad];
-                const double q_prev = k == 0 ? q_last : qj[(k + kRingPad - 1) % kRingPad];
-                const d3 n_next = cross_f(e_next, e[k]);
-                acc += slot_gradient_at<T>(p, g, qv, e_prev, e[k], q_prev, qj[k], n_cur, n_next);
-                n_cur = n_next;
-            }
-        }
-        return acc;
-    }
     const int j0 = m.ring[v], jl = m.ring[(n - 1) * N + v];
     const d3 e_first = load_p(m.x, j0) - xv;
     const double q_first = q[j0];
@@ -1358,6 +1433,7 @@ __device__ __forceinline__ d3 ring_gradient(const MeshView& m, const EParams& p,
     return acc;
 }
 
+// Pass 2 over the kThreads vertices of block lb, one lane per vertex.
 template <class T, int kThreads>
 __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
 {
@@ -1365,23 +1441,26 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
     const EParams& p = A.p;
     const int v = lb * kThreads + threadIdx.x;
     const bool active = v < m.nv;
-    const int meta = active ? m.meta[v] : 0;
+    // every lane loads (lanes past the last vertex a valid one, discarded)
+    const int vc = active ? v : m.nv - 1;
+    const int meta = m.meta[vc];
+    const d3 xv = load_p(m.x, vc);
+    const double qv = A.q[vc];
+    const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
-    d3 acc{0, 0, 0};
-    if (ring) {
-        acc = ring_gradient<T>(m, p, *A.globals, A.q, v, meta_valence(meta), load_p(m.x, v), A.q[v]);
-        // repulsion gradient of the vertices the query found pairs for
-        // (written by the query kernel of this evaluation)
-        if (A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0)
-            acc += d3{A.rep_g[3 * v], A.rep_g[3 * v + 1], A.rep_g[3 * v + 2]};
-    }
+    d3 acc = ring_gradient_short<T>(m, p, A.globals, A.q, vc, ring && n <= kRingPad, n, xv, qv);
+    if (ring && n > kRingPad) acc = ring_gradient_walk<T>(m, p, *A.globals, A.q, v, n, xv, qv);
+    // repulsion gradient of the vertices the query found pairs for (written
+    // by the query kernel of this evaluation)
+    if (ring && A.rep_v && p.kappa_r != 0 && A.pair_cnt[v] > 0)
+        acc += d3{A.rep_g[3 * v], A.rep_g[3 * v + 1], A.rep_g[3 * v + 2]};
     // stage the block's gradient rows and store them as contiguous doubles;
     // rows of listed (boundary) vertices are left to k_generic_grad
-    __shared__ double s_g[3 * kThreads];
+    __shared__ double s_gr[3 * kThreads];
     __shared__ int s_ring[kThreads];
-    s_g[3 * threadIdx.x] = acc.x;
-    s_g[3 * threadIdx.x + 1] = acc.y;
-    s_g[3 * threadIdx.x + 2] = acc.z;
+    s_gr[3 * threadIdx.x] = acc.x;
+    s_gr[3 * threadIdx.x + 1] = acc.y;
+    s_gr[3 * threadIdx.x + 2] = acc.z;
     s_ring[threadIdx.x] = ring;
     __syncthreads();
     const int v0 = lb * kThreads;
@@ -1389,7 +1468,7 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
     for (int c = 0; c < 3; c++) {
         const int i = c * kThreads + threadIdx.x;
         const int r = i / 3;
-        if (v0 + r < m.nv && s_ring[r]) A.grad[3 * v0 + i] = s_g[i];
+        if (v0 + r < m.nv && s_ring[r]) A.grad[3 * v0 + i] = s_gr[i];
     }
 }
 
@@ -1501,13 +1580,22 @@ static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs
     else p2_threads<double>(t.p2_threads, grouped, a, g, s);
 }
 
-// The repulsion query (its own launch: a chain of dependent loads with few
-// registers), then pass 1, which reads the query's rep_v.
+// Pass 1 and the repulsion query (its own launch: a chain of dependent loads
+// with few registers).  The full query, and a recording the host asks for,
+// run first and pass 1 reads their rep_v; the cached query runs after pass
+// 1, which measures the displacements its replay decision needs, and adds
+// column 5 of the partials itself.
 void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
 {
     if (a.m.nv == 0) return;
-    launch_rep(a, t, s);
     const dim3 grid(a.nblocks), block(kPass1Verts);
+    if (a.rep_v && a.pc.x_snap && !a.pc.force) {
+        if (mixed) hipLaunchKernelGGL(k_pass1_apart<float>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(k_pass1_apart<double>, grid, block, 0, s, a);
+        launch_rep(a, t, s);
+        return;
+    }
+    launch_rep(a, t, s);
     if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
     else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
 }
@@ -1530,7 +1618,6 @@ void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s)
         }
         return;
     }
-    hipLaunchKernelGGL(k_disp, dim3(kDispBlocks), dim3(256), 0, s, a.m.x, a.pc.x_snap, a.m.nv, a.pc.disp_part);
     const dim3 grid(nblocks(nblocks(a.m.nv, kVertsPerBlock), kXcds) * kXcds);
     switch (t.rep_batch) {
     case 1: hipLaunchKernelGGL(k_rep_cached<1>, grid, block, 0, s, a); break;
@@ -1553,7 +1640,7 @@ void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int 
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
     launch_pass1_body(a, mixed, t, s);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, a);
 }
 
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
@@ -1572,7 +1659,6 @@ void launch_rep_grouped(const GroupLaunch& g, hipStream_t s)
     for (int r = 0; r < g.R; r++) rep = rep || g.h_args[r].rep_v;
     if (!rep) return;
     if (g.total_rc > 0) {
-        hipLaunchKernelGGL(k_disp_grouped, dim3(g.total_disp), dim3(256), 0, s, g.d_args, g.d_block0_disp, g.R);
         const dim3 grid(g.total_rc), block(256);
         switch (g.tune.rep_batch) {
         case 1: hipLaunchKernelGGL(k_rep_cached_grouped<1>, grid, block, 0, s, g.d_args, g.d_block0_rc, g.R); break;
@@ -1591,11 +1677,18 @@ void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
         if (g.h_args[r].nlist > 0)
             hipLaunchKernelGGL(k_generic_props_at, dim3(nblocks(g.h_args[r].nlist, 128)), dim3(128), 0, s, g.d_args,
                                r);
-    launch_rep_grouped(g, s);
     const dim3 grid(g.total_p1), block(kPass1Verts);
-    if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-    hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(1024), 0, s, g.d_args);
+    if (g.total_rc > 0) {
+        // the cached query after pass 1, as for single meshes
+        if (g.mixed) hipLaunchKernelGGL(k_pass1_apart_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        else hipLaunchKernelGGL(k_pass1_apart_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        launch_rep_grouped(g, s);
+    } else {
+        launch_rep_grouped(g, s);
+        if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+    }
+    hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(64), 0, s, g.d_args);
 }
 
 void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s)

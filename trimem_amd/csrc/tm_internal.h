@@ -19,7 +19,6 @@ namespace tmh {
 // are bit-identical.
 constexpr int kPairLanes = 8;      // lanes per vertex of the query
 constexpr int kPairCap = 4;        // recorded candidates per lane (more: that vertex takes the full query)
-constexpr int kDispBlocks = 128;   // blocks of k_disp, one partial each
 constexpr double kPairSkin = 0.2;  // skin = 0.2 lc1
 // Replay while every vertex has moved less than 0.25 skin from the snapshot
 // of the last recording of all lists.  A vertex that left the cell its list
@@ -32,7 +31,7 @@ struct PairCache {
     int* cell;              // [N] cell id at the recording, -1: replay not allowed
     unsigned* cnt;          // [N] recorded candidates of each lane, 4 bits per lane
     int* ent;               // [8 kPairCap N] ids, lane-major per vertex
-    double* disp_part;      // [disp_n] max |x - x_snap|^2 per block of k_disp
+    double* disp_part;      // [disp_n] max |x - x_snap|^2 per pass-1 block (written by pass 1)
     int disp_n;
     int* dforce;            // replica groups: device flag, nonzero records (the reduction clears it)
     double skin;
@@ -120,10 +119,8 @@ struct GroupLaunch {
     int total_p2;
     Tuning tune;
     int mixed;
-    // with the pair cache (h_args[r].pc.x_snap set): the displacement check
-    // and the cached query, one prefix table each
-    const int* d_block0_disp;
-    int total_disp;
+    // with the pair cache (h_args[r].pc.x_snap set): the cached query's
+    // prefix table (pass-1 blocks of each replica)
     const int* d_block0_rc;
     int total_rc;
 };
