@@ -104,6 +104,17 @@ def cpu_baseline(v, f, params, budget_s):
     return 1.0 / float(np.median(times)), len(times), O.num_threads()
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_flip_baseline(v, f, params, ratio=0.1, seed=7):
     """The oracle's serial flip sweep (flips.cpp:22-80, one thread) on the
     same mesh: one sweep of ratio * edges proposals, injected uniformly at
@@ -278,15 +289,25 @@ def flip_rates(v, f, params, precision, ratio=0.1, sweeps=3):
     return out
 
 
-def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=20261205):
-    """The combined configuration's meshes on one GPU: 32 icosphere L7 meshes
+def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=20261205, chain_steps=20, swap_every=5,
+                   time_step=5e-6, flip_ratio=0.2, guarded_steps=16):
+    """The combined configuration on one GPU: 32 icosphere L7 meshes
     (163842 vertices each, jittered 1% from their own seeds), volume
-    fractions U[0.55, 0.9], kappa_b log-uniform on [20, 80]; the rate of
-    grouped energy+gradient evaluations of all of them and the algorithmic
-    HBM rate (their working set is several times the Infinity Cache)."""
+    fractions U[0.55, 0.9], kappa_b log-uniform on [20, 80] as a tempering
+    ladder.
+
+    * grouped energy+gradient evaluations of all 32 (every kernel one launch
+      over all meshes) and their algorithmic HBM rate;
+    * the Monte Carlo chain: each step is one grouped device HMC step of
+      every replica (10 leapfrog steps, device draws) followed by a parallel
+      flip sweep over 20% of the edges of every replica; a kappa_b exchange
+      round every swap_every steps;
+    * the surrogate's plain-gradient fallback on the first replica: guarded
+      HMC steps driven by an untrained network, which the policy (window 5,
+      acceptance 0.5) replaces by the exact gradient."""
     from trimem_amd import capi, core, workloads
     L = capi.lib()
-    work, _ = workloads.combined(n_replicas, seed)
+    work, ladder = workloads.combined(n_replicas, seed)
     ms, es = [], []
     for v, f, params in work:
         m = core.TriMesh(v, f)
@@ -298,7 +319,8 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
     tune = R.autotune()
     nv = np.array([m.n_vertices() for m in ms], dtype=np.float64)
     nf = np.array([m.n_faces() for m in ms], dtype=np.float64)
-    eval_bytes = float(np.sum((24.0 * nv + 12.0 * nf + 16.0 * nv) * 2 + 24.0 * nv))
+    per_vertex = 104.0 if precision == "fp64" else 52.0    # measurement model, B = per_vertex N + 24 F
+    eval_bytes = float(np.sum(per_vertex * nv + 24.0 * nf))
     R.eval()
     capi.check(L.tm_sync())
     e0, e1 = hip.event(), hip.event()
@@ -307,14 +329,66 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
         R.eval()
     hip.record(e1, stream)
     ms_group = hip.elapsed_ms(e0, e1) / evals
-    return {"replicas": n_replicas, "vertices": int(nv.sum()), "group_evals_per_s": 1e3 / ms_group,
-            "ms_per_group_eval": ms_group, "replica_evals_per_s": n_replicas * 1e3 / ms_group,
-            "algorithmic_bytes_per_group_eval": eval_bytes, "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9,
-            "hbm_frac": eval_bytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "pair_cache")},
-            "workload": "32 icosphere L7 meshes jittered 1% (own seeds), vfrac U[0.55,0.9], kappa_b "
-                        "log-uniform [20,80], cell-list repulsion excl 2 rc=1.5l; one grouped energy+gradient "
-                        "evaluation of all"}
+    out = {"replicas": n_replicas, "vertices": int(nv.sum()), "group_evals_per_s": 1e3 / ms_group,
+           "ms_per_group_eval": ms_group, "replica_evals_per_s": n_replicas * 1e3 / ms_group,
+           "algorithmic_bytes_per_group_eval": eval_bytes, "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9,
+           "hbm_frac": eval_bytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+           "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "pair_cache")},
+           "workload": "32 icosphere L7 meshes jittered 1% (own seeds), vfrac U[0.55,0.9], kappa_b "
+                       "log-uniform [20,80], cell-list repulsion excl 2 rc=1.5l; one grouped energy+gradient "
+                       "evaluation of all"}
+
+    # the chain: grouped HMC + flip sweeps + exchanges
+    R.hmc_setup(mass=1.0, time_step=time_step, temperature=1.0, seeds=list(range(1, n_replicas + 1)),
+                num_integration_steps=10)
+    R.set_ladder(list(ladder), seed=11)
+    R.hmc_run(1)                  # warm-up (graph-free, every kernel shape loaded)
+    capi.check(L.tm_sync())
+    st0 = R.stats()
+    t_hmc = t_flip = 0.0
+    flips_acc, flips_tried = 0, 0
+    for k in range(chain_steps):
+        t0 = time.perf_counter()
+        R.hmc_run(1, swap_every=1 if (k + 1) % swap_every == 0 else 0)
+        capi.check(L.tm_sync())
+        t1 = time.perf_counter()
+        for i, (m, e) in enumerate(zip(ms, es)):
+            flips_acc += core.pflip(m, e, flip_ratio, seed=1000 * k + i)
+            flips_tried += int(m.n_edges() * flip_ratio)
+        capi.check(L.tm_sync())
+        t_hmc += t1 - t0
+        t_flip += time.perf_counter() - t1
+    st = R.stats()
+    acc = float((st["accepted"] - st0["accepted"]).sum() / max((st["steps"] - st0["steps"]).sum(), 1))
+    out["chain"] = {"steps": chain_steps, "steps_per_s": chain_steps / (t_hmc + t_flip),
+                    "hmc_group_steps_per_s": chain_steps / t_hmc,
+                    "flip_attempts_per_s": flips_tried / t_flip,
+                    "hmc_acceptance": acc,
+                    "swap_acceptance": float(st["swaps_accepted"].sum() / max(st["swaps_tried"].sum(), 1)),
+                    "flip_acceptance": flips_acc / max(flips_tried, 1),
+                    "setup": f"HMC dt {time_step}, 10 leapfrog steps, T 1, device draws; parallel flip sweep of "
+                             f"{flip_ratio:.0%} of the edges of every replica after every HMC step; one kappa_b "
+                             f"exchange round every {swap_every} steps"}
+
+    # the surrogate's fallback on replica 0
+    sg = core.Surrogate(ms[0], es[0], seed=3)
+    sg.set_policy(window=5, min_acceptance=0.5, probe_every=5)
+    d = core.DeviceHMC(ms[0], es[0])
+    drivers, accepted = [], []
+    t0 = time.perf_counter()
+    for k in range(guarded_steps):
+        drivers.append(d.step(mass=1.0, time_step=time_step, num_integration_steps=10, temperature=1.0, seed=77,
+                              step=k, surrogate=sg, guarded=True))
+        accepted.append(bool(d.result()["accepted"]))
+    dt_g = time.perf_counter() - t0
+    pol = sg.policy_state()
+    out["surrogate_fallback"] = {
+        "steps": guarded_steps, "drivers": drivers, "accepted": accepted, "steps_per_s": guarded_steps / dt_g,
+        "surrogate_steps": pol["surrogate_steps"], "exact_steps": pol["exact_steps"],
+        "falling_back": pol["falling_back"], "switches": pol["switches"],
+        "exact_acceptance": float(np.mean([a for dr, a in zip(drivers, accepted) if dr == "exact"] or [0.0])),
+        "setup": "replica 0, untrained network, policy window 5 / min acceptance 0.5 / probe every 5 steps"}
+    return out
 
 
 def ess(x):
@@ -486,6 +560,13 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
         lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
                      f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
                      f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
+    if "k_pass2" in stats:
+        nv, nf = 163842, 327680
+        b2 = 32.0 * nv + 12.0 * nf    # mixed pass 2: positions 12, faces 12/face, a,c 8, gradient 12 B/vertex
+        us = stats["k_pass2"]["avg_us"]
+        lines += ["", f"Roofline of k_pass2 (the longest kernel): {b2:.0f} algorithmic bytes per launch "
+                      f"(32 N + 12 F, N = {nv}, F = {nf}) / {us:.2f} us = {b2 / us / 1e3:.0f} GB/s, "
+                      f"{b2 / us / 1e3 / HBM_PEAK_GBPS:.3f} of the 8 TB/s peak."]
     p1 = [k for k in PASS1_KERNELS if k in stats]
     if p1:
         lines += ["", f"Pass 1 ({' + '.join(p1)}): {sum(stats[k]['avg_us'] for k in p1):.2f} us, "
@@ -527,6 +608,10 @@ def main():
                     help="TRACE_DIR FETCH_DIR WRITE_DIR [SQ_DIR]: write profiles/ summaries from the "
                          "rocprofv3 output folders and exit")
     args = ap.parse_args()
+    # the OpenMP oracle (loaded later) pinned one thread per core, close
+    # together, as the measurement model asks for the CPU baseline
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.summarize:
         only = " --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0 --flips 0 --combined 0"
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
@@ -589,7 +674,10 @@ def main():
     ms_total = hip.elapsed_ms(e0, e1)
     evals_per_s = args.steps / (ms_total / 1e3)
 
-    # per-kernel durations, events around each pass on the same stream
+    # per-launch durations, events on the library stream around each: the
+    # gradient pass is one kernel (k_pass2; the mesh is closed, so no
+    # boundary walker follows it), the properties pass the launch sequence
+    # k_pass1_apart + k_rep_cached + k_finalize
     ks = min(args.steps, 500)
     evs = [[hip.event() for _ in range(3)] for _ in range(ks)]
     for i in range(ks):
@@ -604,47 +692,53 @@ def main():
     capi.check(L.tm_energy_eval_device(em, mesh, both))
     capi.check(L.tm_sync())
 
+    def timed_evals(k):
+        for _ in range(args.warmup):
+            L.tm_energy_eval_device(em, mesh, both)
+        hip.record(e0, stream)
+        for _ in range(k):
+            L.tm_energy_eval_device(em, mesh, both)
+        hip.record(e1, stream)
+        capi.check(L.tm_sync())
+        return k / (hip.elapsed_ms(e0, e1) / 1e3)
+
     # the same evaluations with the pair cache off (the full cell-list query
     # every time), for comparison; the headline keeps the default
     tun = capi.tm_tuning()
     capi.check(L.tm_energy_get_tuning(em, ctypes.byref(tun)))
     tun.pair_cache = 0
     capi.check(L.tm_energy_set_tuning(em, ctypes.byref(tun)))
-    for _ in range(args.warmup):
-        L.tm_energy_eval_device(em, mesh, both)
-    hip.record(e0, stream)
-    for _ in range(args.steps):
-        L.tm_energy_eval_device(em, mesh, both)
-    hip.record(e1, stream)
-    capi.check(L.tm_sync())
-    full_query_evals_per_s = args.steps / (hip.elapsed_ms(e0, e1) / 1e3)
+    full_query_evals_per_s = timed_evals(args.steps)
     tun.pair_cache = 1
     capi.check(L.tm_energy_set_tuning(em, ctypes.byref(tun)))
+    # the other precision path on the same mesh
+    other = "fp64" if args.precision == "mixed" else "mixed"
+    capi.check(L.tm_energy_set_precision(em, capi.TM_PREC_FP64 if other == "fp64" else capi.TM_PREC_MIXED))
+    other_evals_per_s = timed_evals(args.steps)
+    capi.check(L.tm_energy_set_precision(em, capi.TM_PREC_MIXED if args.precision == "mixed" else capi.TM_PREC_FP64))
 
-    # algorithmic bytes per launch: pass 1 reads positions (24 B/vertex) and
-    # the triangle table (12 B/face) and writes a_i, c_i (16 B/vertex); pass 2
-    # reads the same plus a_i, c_i and writes the gradient (24 B/vertex); the
-    # positions are FP64 in both precision paths
-    b_pass1 = 24.0 * n + 12.0 * nf + 16.0 * n
-    b_pass2 = b_pass1 + 24.0 * n
+    # algorithmic bytes per launch, the measurement model's per-unit figures:
+    # pass 1 reads positions and the triangle table (12 B/face) and writes
+    # a_i, c_i; pass 2 reads the same plus a_i, c_i and writes the gradient.
+    # Positions, a_i/c_i and the gradient count 24/16/24 B per vertex in the
+    # FP64 path and 12/8/12 B in the mixed path (FP32 bulk), whatever width
+    # the kernels move them in.
+    wp, wq, wg = (24.0, 16.0, 24.0) if args.precision == "fp64" else (12.0, 8.0, 12.0)
+    b_pass1 = wp * n + 12.0 * nf + wq * n
+    b_pass2 = wp * n + 12.0 * nf + wq * n + wg * n
     t1, t2 = float(np.mean(p1)) * 1e-3, float(np.mean(p2)) * 1e-3
-    # pass 1 is four launches on the stream (pair-cache displacement check,
-    # repulsion query, ring properties, fixed-order global sums), timed
-    # together; pass 2 is one
-    phases = {"pass1": PASS1_KERNELS, "pass2": ("k_pass2",)}
-    dom = ("pass2", b_pass2, t2) if t2 >= t1 else ("pass1", b_pass1, t1)
-    achieved = dom[1] / dom[2] / 1e9
+    achieved = b_pass2 / t2 / 1e9
     eval_bytes = b_pass1 + b_pass2
-    traffic, valu_frac = None, None
+    traffic, valu_frac, prof_us = None, None, None
     if os.path.exists(PROFILE_JSON) and args.level == 7 and args.precision == "mixed":
         prof = json.load(open(PROFILE_JSON))
-        ks_ = [prof["kernels"].get(k) for k in phases[dom[0]]]
-        if all(k is not None for k in ks_):
-            traffic = sum(k["hbm_bytes_per_dispatch"] for k in ks_)
-            if all("valu_issue_us" in k for k in ks_):
-                # time the phase's VALU instructions take to issue / its duration
-                valu_frac = sum(k["valu_issue_us"] for k in ks_) / (dom[2] * 1e6)
-
+        k2 = prof["kernels"].get("k_pass2")
+        if k2 is not None:
+            traffic = k2["hbm_bytes_per_dispatch"]
+            prof_us = k2.get("avg_us")
+            if "valu_issue_us" in k2:
+                # the time the kernel's VALU instructions take to issue / its duration
+                valu_frac = k2["valu_issue_us"] / (t2 * 1e6)
     out = {
         "metric": "energy+grad evals/s",
         "metric_full": METRIC_FULL,
@@ -664,13 +758,17 @@ def main():
                                "edge tether, cell-list repulsion excl 2 rc=1.5l; one step = energy+gradient",
                    "precision": args.precision, "vertices": n, "faces": nf},
         "hbm_gbps_per_eval": eval_bytes * evals_per_s / 1e9,
-        "kernels_ms": {"pass1": float(np.mean(p1)), "pass2": float(np.mean(p2))},
-        "roofline": {"kernel": dom[0] + " (" + " + ".join(phases[dom[0]]) + ")", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "algorithmic_bytes_per_eval": eval_bytes,
+        "kernels_ms": {"pass1_phase": t1 * 1e3, "k_pass2": t2 * 1e3},
+        "roofline": {"kernel": "k_pass2", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "valu_issue_frac": valu_frac,
-                     "algorithmic_bytes": dom[1]},
+                     "algorithmic_bytes": b_pass2, "duration_us": t2 * 1e6, "rocprof_avg_us": prof_us,
+                     "valu_issue_frac": valu_frac},
+        "pass1_phase": {"kernels": list(PASS1_KERNELS), "duration_us": t1 * 1e6, "algorithmic_bytes": b_pass1,
+                        "achieved_gbps": b_pass1 / t1 / 1e9, "frac": b_pass1 / t1 / 1e9 / HBM_PEAK_GBPS},
         "wall_s_timed": wall,
         "full_query_evals_per_s": full_query_evals_per_s,
+        f"{other}_evals_per_s": other_evals_per_s,
     }
     if args.hmc_steps > 0:
         out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))
@@ -686,7 +784,10 @@ def main():
         cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",
                                "sample": f"{ns} FP64 energy()+gradient() evals of the same L{args.level} mesh, "
-                                         f"median, OpenMP {thr} threads, {platform.processor() or 'host'}"}
+                                         f"median, OpenMP {thr} threads",
+                               "cpu": cpu_model(), "host_cpus": os.cpu_count(),
+                               "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
+                                                                      "OMP_PLACES")}}
         out["speedup_vs_cpu"] = evals_per_s / cv
         if args.flips:
             out["cpu_baseline"]["flips"] = cpu_flip_baseline(v, f, params)

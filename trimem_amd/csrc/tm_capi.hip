@@ -1527,6 +1527,7 @@ struct tm_replicas {
     void force(int r) { h2d(force_flags.p + r, h_ones.data() + r, sizeof(int)); }
     // HMC
     bool hmc_ready = false;
+    long long steps_run = 0;    // HMC steps run since the setup
     int L = 0;
     std::vector<HmcArgs> h_hmc;
     DevBuf<HmcArgs> d_hmc;
@@ -1798,6 +1799,8 @@ int tm_replicas_hmc_setup(tm_replicas* rs, const double* mass, const double* tim
         h.dt = time_step[r];
         h.dt2 = time_step[r] * time_step[r];
         h.temperature = temperature[r];
+        require(!S.has_ladder || temperature[r] == temperature[0],
+                "replica exchange over a kappa_b ladder needs one temperature for all replicas");
         h.seed = seeds[r];
         S.h_hmc[r] = h;
         S.h_hblock0[r + 1] = S.h_hblock0[r] + h.nblocks;
@@ -1819,6 +1822,11 @@ int tm_replicas_set_ladder(tm_replicas* rs, const double* ladder, uint64_t seed)
     tm_replicas& S = *rs;
     S.h_ladder.assign(ladder, ladder + S.R);
     for (int s = 1; s < S.R; s++) require(S.h_ladder[s] >= S.h_ladder[s - 1], "the ladder must be ascending");
+    // the exchange rule min(1, exp(-dk dB / T)) assumes one temperature
+    if (S.hmc_ready)
+        for (int r = 1; r < S.R; r++)
+            require(S.h_hmc[r].temperature == S.h_hmc[0].temperature,
+                    "replica exchange over a kappa_b ladder needs one temperature for all replicas");
     S.ladder.alloc(S.R);
     S.owner.alloc(S.R);
     S.pt_round.alloc(1);
@@ -1842,6 +1850,9 @@ int tm_replicas_swap(tm_replicas* rs)
     API_BEGIN
     tm_replicas& S = *rs;
     require(S.has_ladder && S.hmc_ready, "exchange needs a ladder and HMC parameters");
+    // the exchange compares the bending of each replica's current state,
+    // which an HMC step records
+    require(S.steps_run > 0, "exchange needs at least one HMC step first");
     replicas_current(S);
     launch_pt_swap(S.d_args.p, S.d_hmc.p, S.owner.p, S.ladder.p, S.R, S.pt_seed, S.pt_round.p, S.swaps_acc.p,
                    S.swaps_tried.p, g_rt.stream);
@@ -1874,6 +1885,7 @@ int tm_replicas_hmc_run(tm_replicas* rs, int n_steps, int swap_every, int use_gr
                            S.swaps_acc.p, S.swaps_tried.p, g_rt.stream);
     }
     check_launch();
+    S.steps_run += n_steps;
     for (tm_mesh* M : S.meshes) M->device_ahead = true;
     API_END
 }
