@@ -475,24 +475,48 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     }
     launch_box(M.x.p, n, L.box_part.p, L.box_counter.p, L.box.p, g_rt.stream);
     check_launch();
-    double box[6];
-    d2h(box, L.box.p, sizeof(box));
+    double tight[6];    // the vertices' own bounding box
+    d2h(tight, L.box.p, sizeof(tight));
     sync();
+    // the reference's box (cell_list.h:46-56): its bounds start at min =
+    // (+max, 0, 0) and max = (-1, 0, 0), so it spans 0 in y and z
+    double box[6];
+    for (int k = 0; k < 3; k++) {
+        box[k] = k == 0 ? tight[0] : std::min(0.0, tight[k]);
+        box[3 + k] = std::max(k == 0 ? -1.0 : 0.0, tight[3 + k]);
+    }
     const double box_eps = 1.0e-6;
     NListView v{};
     v.kind = 0;
     v.exclusion = L.exclusion;
     long long ncells = 1;
     for (int k = 0; k < 3; k++) {
-        double dim = box[3 + k] - box[k] + box_eps;
-        int nk = int(dim / L.rlist);
+        if (n == 0) {
+            v.box_min[k] = 0.0;
+            v.r_list[k] = 1.0;
+            v.lo[k] = 0;
+            v.shape[k] = 1;
+            continue;
+        }
+        const double dim = box[3 + k] - box[k] + box_eps;
+        const double q = dim / L.rlist;
+        if (!(q < 1073741824.0))
+            throw TmError(TM_E_UNSUPPORTED, "cell grid too fine along axis " + std::to_string(k) + " (" +
+                                                std::to_string(q) + " cells); coordinates must be finite");
+        const int nk = int(q);
+        const int gk = (nk > 0) ? nk : 1;
         v.box_min[k] = box[k];
         v.r_list[k] = (nk > 0) ? dim / nk : dim;
-        v.shape[k] = (nk > 0) ? nk : 1;
+        // global cells of the lowest and highest vertex; the cell index is
+        // monotone in the coordinate, so they bound every vertex's cell
+        auto cell = [&](double p) { return std::min(std::max(int((p - box[k]) / v.r_list[k]), 0), gk - 1); };
+        v.lo[k] = cell(tight[k]);
+        v.shape[k] = cell(tight[3 + k]) - v.lo[k] + 1;
         ncells *= v.shape[k];
     }
     if (ncells > (1ll << 28))
-        throw TmError(TM_E_UNSUPPORTED, "cell grid too large (" + std::to_string(ncells) + " cells)");
+        throw TmError(TM_E_UNSUPPORTED, "occupied cell sub-grid too large (" + std::to_string(ncells) +
+                                            " cells): the mesh spans more than 2^28 cells of size rlist");
     v.strides[0] = 1;
     v.strides[1] = v.shape[0];
     v.strides[2] = v.shape[0] * v.shape[1];

@@ -89,9 +89,12 @@ struct MeshView {
 struct NListView {
     int kind;
     int exclusion;
-    // cell list
+    // cell list.  Cell coordinates are the reference's, int((p - box_min) /
+    // r_list) per dimension; the table covers only the occupied sub-grid,
+    // global coordinates lo .. lo + shape - 1, and stores them relative to lo
     double box_min[3];
     double r_list[3];
+    int lo[3];
     int shape[3];
     int strides[3];
     const int* __restrict__ cell_start;    // [ncells + 1]

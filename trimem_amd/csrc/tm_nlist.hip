@@ -9,8 +9,13 @@
 // FP64 from the reduced box), flat cell ids sum_k int((p_k - min_k) / r_k)
 // * stride_k (cell_list.h:70-79), then a counting sort: histogram, exclusive
 // scan, scatter, and a per-cell sort by vertex id so that the order inside a
-// cell is ascending like the reference's push_back order.  The cell table is
-// dense over the grid (cell_start has ncells + 1 entries).
+// cell is ascending like the reference's push_back order.  The reference keys
+// its cells in a map (cell_list.h:31), so only occupied cells cost memory;
+// here the table is dense over the occupied sub-grid, the global cells lo ..
+// hi that hold a vertex at build time, which is the mesh's own bounding box
+// in cells whatever the reference's box (anchored at 0 in y and z) spans.
+// Stencils clipped to the sub-grid visit the same vertices as stencils
+// clipped to the global grid: every cell outside the sub-grid is empty.
 //
 // Queries use the current positions with the cells frozen at build time,
 // exactly as the reference does between two update_repulsion calls.  All
@@ -25,17 +30,18 @@
 namespace tmh {
 
 // ----------------------------------------------------------------- box ----
-// Bounding box with the reference's initial values (cell_list.h:46-47):
-// min = (+DBL_MAX, 0, 0), max = (-1, 0, 0).  Grid-stride partial min/max per
-// block, and the last block to finish combines the partials.
+// Bounding box of the vertices: min = (+DBL_MAX)^3 and max = (-DBL_MAX)^3
+// for no vertex.  The host applies the reference's initial values
+// (cell_list.h:46-47) to it.  Grid-stride partial min/max per block, and the
+// last block to finish combines the partials.
 constexpr int kBoxBlocks = 64;
 __global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n, double* __restrict__ part,
                                               unsigned* __restrict__ counter, double* __restrict__ box)
 {
     __shared__ double s[6][4];
     __shared__ int s_last;
-    double mn[3] = {DBL_MAX, 0.0, 0.0};
-    double mx[3] = {-1.0, 0.0, 0.0};
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         for (int k = 0; k < 3; k++) {
             double v = x[3 * i + k];
@@ -128,14 +134,15 @@ void launch_box(const double* x, int n, double* scratch, unsigned* counter, doub
 }
 
 // ------------------------------------------------------------- binning ----
-// Build-time cell of vertex v.  The coordinate is clamped into the grid: the
-// reference's box epsilon (1e-6) vanishes in rounding for coordinates beyond
-// ~1e9, where int((max - min) / r) can reach the grid size.
+// Build-time cell of vertex v in the occupied sub-grid.  The coordinate is
+// clamped into the sub-grid: the reference's box epsilon (1e-6) vanishes in
+// rounding for coordinates beyond ~1e9, where int((max - min) / r) can reach
+// the grid size.
 __device__ __forceinline__ int flat_cell(const NListView& nl, const double* x, int v)
 {
     int id = 0;
     for (int k = 0; k < 3; k++) {
-        int c = (int)((x[3 * v + k] - nl.box_min[k]) / nl.r_list[k]);
+        int c = cell_coord(nl, x[3 * v + k], k);
         c = min(max(c, 0), nl.shape[k] - 1);
         id += c * nl.strides[k];
     }

@@ -268,12 +268,22 @@ __device__ __forceinline__ bool om_excluded(const TP& t, int i, int j, int excl)
     return false;
 }
 
-// Cell coordinates of a position (cell_list.h:76, 141).
+// Cell coordinates of a position (cell_list.h:76, 141), relative to the
+// occupied sub-grid.  The quotient is clamped to two cells beyond the
+// sub-grid before the conversion: every stencil from there on is empty, as
+// it is for the reference's unclamped coordinate, and the conversion stays
+// defined for a vertex that has moved arbitrarily far (or is not finite).
+__device__ __forceinline__ int cell_coord(const NListView& nl, double p, int k)
+{
+    const double q = (p - nl.box_min[k]) / nl.r_list[k];
+    return (int)fmin(fmax(q, (double)(nl.lo[k] - 2)), (double)(nl.lo[k] + nl.shape[k] + 1)) - nl.lo[k];
+}
+
 __device__ __forceinline__ void cell_coords(const NListView& nl, d3 p, int c[3])
 {
-    c[0] = (int)((p.x - nl.box_min[0]) / nl.r_list[0]);
-    c[1] = (int)((p.y - nl.box_min[1]) / nl.r_list[1]);
-    c[2] = (int)((p.z - nl.box_min[2]) / nl.r_list[2]);
+    c[0] = cell_coord(nl, p.x, 0);
+    c[1] = cell_coord(nl, p.y, 1);
+    c[2] = cell_coord(nl, p.z, 2);
 }
 
 // Visit every neighbour j of vertex i within dmax (point_distances of
