@@ -490,7 +490,7 @@ def _short(name):
     for k in ("k_rep_cached", "k_rep_record", "k_rep", "k_pass1_apart", "k_pass1", "k_pass2",
               "k_finalize", "k_generic_props", "k_generic_grad"):
         if k in name:
-            return k + ("_fp64" if "<double>" in name else "")
+            return k + ("_fp64" if "<double" in name else "")
     return None
 
 
@@ -587,6 +587,55 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
     print("\n".join(lines))
 
 
+SG_COUNTERS = ("SQ_INSTS_VALU_MFMA_MOPS_BF16", "MfmaUtil")
+BF16_MFMA_PEAK_TFLOPS = 2500.0    # MI355X dense bf16
+
+
+def _sg_name(name):
+    """k_sg_* kernel name with its template arguments, or None."""
+    i = name.find("k_sg_")
+    if i < 0:
+        return None
+    j = name.find("(", i)
+    return name[i:j if j > 0 else None]
+
+
+def summarize_surrogate(trace_dir, pmc_dir, out_md, command):
+    """Summary of the surrogate network's kernels: durations (--kernel-trace
+    --stats) and, per dispatch, the bf16 MFMA work (--pmc
+    SQ_INSTS_VALU_MFMA_MOPS_BF16, 512 FLOPs each) and the MFMA busy share
+    of the SIMDs (the derived MfmaUtil)."""
+    stats = {}
+    for r in _csv_rows(_find(trace_dir, "kernel_stats.csv")):
+        k = _sg_name(r["Name"])
+        if k:
+            stats[k] = {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3,
+                        "total_ms": float(r["TotalDurationNs"]) / 1e6}
+    pmc = {}
+    for r in _csv_rows(_find(pmc_dir, "counter_collection.csv")):
+        k = _sg_name(r.get("Kernel_Name", r.get("Kernel-Name", "")))
+        if k:
+            pmc.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    lines = ["# rocprofv3 summary: surrogate network kernels, icosphere L6 (40962 vertices)", "",
+             "Commands (run from the repository root on one MI355X):", "", "```"] + command + ["```", "",
+             "bf16 MFMA work per dispatch from SQ_INSTS_VALU_MFMA_MOPS_BF16 (units of 512 FLOPs), rate over the "
+             f"kernel's average duration, against the {BF16_MFMA_PEAK_TFLOPS:.0f} TFLOP/s dense bf16 peak; "
+             "MfmaUtil is rocprofv3's derived MFMA-busy share of all SIMD cycles while the kernel runs.", "",
+             "| kernel | calls | avg (us) | total (ms) | MFMA GFLOP / dispatch | TFLOP/s | of peak | MfmaUtil (%) |",
+             "|---|---|---|---|---|---|---|---|"]
+    total = sum(v["total_ms"] for v in stats.values())
+    for k in sorted(stats, key=lambda k: -stats[k]["total_ms"]):
+        st, c = stats[k], pmc.get(k, {})
+        fl = float(np.mean(c["SQ_INSTS_VALU_MFMA_MOPS_BF16"])) * 512 if "SQ_INSTS_VALU_MFMA_MOPS_BF16" in c else 0.0
+        util = float(np.mean(c["MfmaUtil"])) if "MfmaUtil" in c else float("nan")
+        tf = fl / (st["avg_us"] * 1e-6) / 1e12 if fl else 0.0
+        lines.append(f"| {k} | {st['calls']} | {st['avg_us']:.2f} | {st['total_ms']:.2f} | {fl / 1e9:.4f} | "
+                     f"{tf:.2f} | {tf / BF16_MFMA_PEAK_TFLOPS:.4f} | {util:.2f} |")
+    lines += ["", f"All surrogate kernels: {total:.2f} ms over the run."]
+    open(out_md, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -600,10 +649,15 @@ def main():
                     help="1: include the surrogate-vs-plain HMC block (L6), 0: skip")
     ap.add_argument("--replica-evals", type=int, default=40,
                     help="grouped evaluations of the 64-replica set timed for the replicas block (0: skip)")
+    ap.add_argument("--sg-train", type=int, default=120, help="surrogate block: training snapshots")
+    ap.add_argument("--sg-epochs", type=int, default=15, help="surrogate block: training epochs")
+    ap.add_argument("--sg-chain", type=int, default=300, help="surrogate block: HMC steps per chain")
     ap.add_argument("--flips", type=int, default=1, help="attempted flips/s of serial and parallel sweeps")
     ap.add_argument("--combined", type=int, default=1, help="grouped evaluations of 32 L7 meshes")
     ap.add_argument("--hmc-steps", type=int, default=40,
                     help="device-resident HMC steps timed for the hmc block (0: skip)")
+    ap.add_argument("--summarize-surrogate", nargs=2, metavar="DIR",
+                    help="TRACE_DIR PMC_DIR: write the surrogate kernels' summary to profiles/ and exit")
     ap.add_argument("--summarize", nargs="+", metavar="DIR",
                     help="TRACE_DIR FETCH_DIR WRITE_DIR [SQ_DIR]: write profiles/ summaries from the "
                          "rocprofv3 output folders and exit")
@@ -612,6 +666,17 @@ def main():
     # together, as the measurement model asks for the CPU baseline
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    if args.summarize_surrogate:
+        sg = (" --steps 10 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --flips 0 --combined 0"
+              " --surrogate 1 --sg-train 40 --sg-epochs 5 --sg-chain 20")
+        cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize_surrogate[0] +
+               " -o run -- python3 bench.py" + sg,
+               "rocprofv3 --pmc " + " ".join(SG_COUNTERS) + " --output-format csv -d " +
+               args.summarize_surrogate[1] + " -o run -- python3 bench.py" + sg,
+               "python3 bench.py --summarize-surrogate " + " ".join(args.summarize_surrogate)]
+        summarize_surrogate(args.summarize_surrogate[0], args.summarize_surrogate[1],
+                            os.path.join(ROOT, "profiles", "surrogate_kernels_L6.md"), cmd)
+        return
     if args.summarize:
         only = " --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0 --flips 0 --combined 0"
         cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + args.summarize[0] +
@@ -773,7 +838,8 @@ def main():
     if args.hmc_steps > 0:
         out["hmc"] = hmc_rates(v, f, params, args.precision, args.hmc_steps, max(2, args.hmc_steps // 20))
     if args.surrogate:
-        out["surrogate"] = surrogate_rates(args.precision)
+        out["surrogate"] = surrogate_rates(args.precision, n_train=args.sg_train, epochs=args.sg_epochs,
+                                           chain=args.sg_chain)
     if args.replica_evals > 0:
         out["replicas"] = replica_rates(args.precision, args.replica_evals, args.hmc_steps, hip, stream)
     if args.flips:
