@@ -329,15 +329,35 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
         R.eval()
     hip.record(e1, stream)
     ms_group = hip.elapsed_ms(e0, e1) / evals
+    # pass 2 of the group alone (the gradient over pass 1's outputs of the
+    # last evaluation), the dominant kernel at this size
+    p2_bytes = float(np.sum((32.0 if precision != "fp64" else 64.0) * nv + 12.0 * nf))
+    hip.record(e0, stream)
+    for _ in range(evals):
+        R.eval(properties=False, gradient=True)
+    hip.record(e1, stream)
+    capi.check(L.tm_sync())
+    us_p2 = hip.elapsed_ms(e0, e1) / evals * 1e3
+    roof = {"kernel": "k_pass2_grouped", "bound": "hbm", "achieved": p2_bytes / us_p2 / 1e3,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": p2_bytes / us_p2 / 1e3 / HBM_PEAK_GBPS,
+            "algorithmic_bytes": p2_bytes, "duration_us": us_p2}
+    gp = os.path.join(ROOT, "profiles", "pmc_energy_32xL7_grouped.json")
+    if os.path.exists(gp) and precision == "mixed" and n_replicas == 32:
+        k2 = json.load(open(gp))["kernels"].get("k_pass2_grouped", {})
+        roof["traffic"] = k2.get("hbm_bytes_per_dispatch")
+        roof["rocprof_median_us"] = k2.get("avg_us")
     out = {"replicas": n_replicas, "vertices": int(nv.sum()), "group_evals_per_s": 1e3 / ms_group,
            "ms_per_group_eval": ms_group, "replica_evals_per_s": n_replicas * 1e3 / ms_group,
            "algorithmic_bytes_per_group_eval": eval_bytes, "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9,
            "hbm_frac": eval_bytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+           "roofline": roof,
            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "pair_cache")},
            "workload": "32 icosphere L7 meshes jittered 1% (own seeds), vfrac U[0.55,0.9], kappa_b "
                        "log-uniform [20,80], cell-list repulsion excl 2 rc=1.5l; one grouped energy+gradient "
                        "evaluation of all"}
 
+    if chain_steps <= 0:
+        return out
     # the chain: grouped HMC + flip sweeps + exchanges
     R.hmc_setup(mass=1.0, time_step=time_step, temperature=1.0, seeds=list(range(1, n_replicas + 1)),
                 num_integration_steps=10)
@@ -490,7 +510,7 @@ def _short(name):
     for k in ("k_rep_cached", "k_rep_record", "k_rep", "k_pass1_apart", "k_pass1", "k_pass2",
               "k_finalize", "k_generic_props", "k_generic_grad"):
         if k in name:
-            return k + ("_fp64" if "<double" in name else "")
+            return k + ("_grouped" if "_grouped" in name else "") + ("_fp64" if "<double" in name else "")
     return None
 
 
@@ -501,17 +521,32 @@ SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD",
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4    # 1024 SIMDs, one wave64 VALU op per 4 cycles at 2.4 GHz
 
 
-def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir=None):
+L7_N, L7_F = 163842, 327680
+# (title, kernel of the roofline line, its algorithmic bytes per launch and
+# how they are made up, the launches of pass 1)
+SINGLE_PROFILE = ("energy + gradient passes, icosphere L7, mixed precision", "k_pass2",
+                  32.0 * L7_N + 12.0 * L7_F, f"32 N + 12 F, N = {L7_N}, F = {L7_F}", PASS1_KERNELS)
+GROUP_PROFILE = ("energy + gradient passes, 32 icosphere L7 meshes in one group, mixed precision",
+                 "k_pass2_grouped", 32 * (32.0 * L7_N + 12.0 * L7_F),
+                 f"32 meshes x (32 N + 12 F), N = {L7_N}, F = {L7_F}",
+                 tuple(k + "_grouped" for k in PASS1_KERNELS))
+
+
+def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir=None, what=None):
     """Summaries of rocprofv3 runs of this script: average kernel durations
     (--kernel-trace --stats) and HBM KiB per dispatch (--pmc FETCH_SIZE, then
     WRITE_SIZE), scaled by what the counters read for the calibration copy
     of COPY_BYTES; with sq_dir, the VALU instructions per dispatch
     (--pmc SQ_COUNTERS) and the time their issue takes."""
-    stats = {}
-    for r in _csv_rows(_find(trace_dir, "kernel_stats.csv")):
-        k = _short(r["Name"])
+    # per kernel the median over its dispatches (a recording launch among
+    # replays, or the autotuner's trial shapes, do not move it)
+    durs = {}
+    for r in _csv_rows(_find(trace_dir, "kernel_trace.csv")):
+        k = _short(r["Kernel_Name"])
         if k:
-            stats[k] = {"calls": int(r["Calls"]), "avg_us": float(r["AverageNs"]) / 1e3}
+            durs.setdefault(k, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    stats = {k: {"calls": len(d), "avg_us": float(np.median(d)), "mean_us": float(np.mean(d))}
+             for k, d in durs.items()}
     pmc, calib = {}, {}
     for d, cname in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
         for r in _csv_rows(_find(d, "counter_collection.csv")):
@@ -529,8 +564,8 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
     f_write = max(calib["WRITE_SIZE"])[1] / copy_kib
     kernels = {}
     for k, c in pmc.items():
-        fk = float(np.mean(c["FETCH_SIZE"]))
-        wk = float(np.mean(c["WRITE_SIZE"]))
+        fk = float(np.median(c["FETCH_SIZE"]))
+        wk = float(np.median(c["WRITE_SIZE"]))
         kernels[k] = {"fetch_kib": fk, "write_kib": wk,
                       "hbm_bytes_per_dispatch": (fk / f_fetch + wk / f_write) * 1024.0,
                       "avg_us": stats.get(k, {}).get("avg_us")}
@@ -542,32 +577,33 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
                 sq.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
         for k, c in sq.items():
             if k in kernels and "SQ_INSTS_VALU" in c:
-                valu = float(np.mean(c["SQ_INSTS_VALU"]))
+                valu = float(np.median(c["SQ_INSTS_VALU"]))
                 kernels[k]["valu_insts"] = valu
                 kernels[k]["valu_issue_us"] = valu / VALU_ISSUE_PER_S * 1e6
-                kernels[k]["waves"] = float(np.mean(c.get("SQ_WAVES", [0])))
+                kernels[k]["waves"] = float(np.median(c.get("SQ_WAVES", [0])))
     out = {"command": command, "copy_bytes": COPY_BYTES, "copy_factor_fetch": f_fetch,
            "copy_factor_write": f_write, "kernels": kernels, "durations": stats}
     json.dump(out, open(out_json, "w"), indent=1)
-    lines = ["# rocprofv3 summary: energy + gradient passes, icosphere L7, mixed precision", "",
+    title, rk, rbytes, rtext, pass1 = what or SINGLE_PROFILE
+    lines = ["# rocprofv3 summary: " + title, "",
              "Commands (run from the repository root on one MI355X):", "", "```"] + command + ["```", "",
              f"Copy calibration: {COPY_BYTES} B device-to-device copy reads FETCH_SIZE x {f_fetch:.4f} "
              f"and WRITE_SIZE x {f_write:.4f} of its size; kernel traffic below is divided by these factors.", "",
-             "| kernel | calls | avg duration (us) | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes / dispatch |",
+             "Durations and counters are medians over each kernel's dispatches.", "",
+             "| kernel | calls | duration (us) | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes / dispatch |",
              "|---|---|---|---|---|---|"]
     for k in sorted(stats):
         kk = kernels.get(k, {})
         lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
                      f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
                      f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
-    if "k_pass2" in stats:
-        nv, nf = 163842, 327680
-        b2 = 32.0 * nv + 12.0 * nf    # mixed pass 2: positions 12, faces 12/face, a,c 8, gradient 12 B/vertex
-        us = stats["k_pass2"]["avg_us"]
-        lines += ["", f"Roofline of k_pass2 (the longest kernel): {b2:.0f} algorithmic bytes per launch "
-                      f"(32 N + 12 F, N = {nv}, F = {nf}) / {us:.2f} us = {b2 / us / 1e3:.0f} GB/s, "
-                      f"{b2 / us / 1e3 / HBM_PEAK_GBPS:.3f} of the 8 TB/s peak."]
-    p1 = [k for k in PASS1_KERNELS if k in stats]
+    if rk in stats:
+        # mixed pass 2 per vertex: positions 12, a and c 8, gradient 12 B; 12 B per face
+        us = stats[rk]["avg_us"]
+        lines += ["", f"Roofline of {rk} (the longest kernel): {rbytes:.0f} algorithmic bytes per launch "
+                      f"({rtext}) / {us:.2f} us = {rbytes / us / 1e3:.0f} GB/s, "
+                      f"{rbytes / us / 1e3 / HBM_PEAK_GBPS:.3f} of the 8 TB/s peak."]
+    p1 = [k for k in pass1 if k in stats]
     if p1:
         lines += ["", f"Pass 1 ({' + '.join(p1)}): {sum(stats[k]['avg_us'] for k in p1):.2f} us, "
                       f"{sum(kernels.get(k, {}).get('hbm_bytes_per_dispatch', 0.0) for k in p1):.0f} HBM bytes."]
@@ -654,6 +690,11 @@ def main():
     ap.add_argument("--sg-chain", type=int, default=300, help="surrogate block: HMC steps per chain")
     ap.add_argument("--flips", type=int, default=1, help="attempted flips/s of serial and parallel sweeps")
     ap.add_argument("--combined", type=int, default=1, help="grouped evaluations of 32 L7 meshes")
+    ap.add_argument("--combined-chain", type=int, default=20,
+                    help="combined block: Monte Carlo steps of the chain (0: evaluations only)")
+    ap.add_argument("--summarize-group", nargs=4, metavar="DIR",
+                    help="TRACE_DIR FETCH_DIR WRITE_DIR SQ_DIR of the combined block's evaluations: write the "
+                         "grouped passes' summary to profiles/ and exit")
     ap.add_argument("--hmc-steps", type=int, default=40,
                     help="device-resident HMC steps timed for the hmc block (0: skip)")
     ap.add_argument("--summarize-surrogate", nargs=2, metavar="DIR",
@@ -666,6 +707,19 @@ def main():
     # together, as the measurement model asks for the CPU baseline
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    if args.summarize_group:
+        grp = (" --steps 10 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --surrogate 0 --flips 0"
+               " --combined 1 --combined-chain 0")
+        d = args.summarize_group
+        cmd = ["rocprofv3 --kernel-trace --stats --output-format csv -d " + d[0] + " -o run -- python3 bench.py" + grp,
+               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d " + d[1] + " -o run -- python3 bench.py" + grp,
+               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d " + d[2] + " -o run -- python3 bench.py" + grp,
+               "rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d " + d[3] +
+               " -o run -- python3 bench.py" + grp,
+               "python3 bench.py --summarize-group " + " ".join(d)]
+        summarize(d[0], d[1], d[2], os.path.join(ROOT, "profiles", "pmc_energy_32xL7_grouped.json"),
+                  os.path.join(ROOT, "profiles", "energy_passes_32xL7_grouped.md"), cmd, d[3], GROUP_PROFILE)
+        return
     if args.summarize_surrogate:
         sg = (" --steps 10 --warmup 2 --no-cpu-baseline --hmc-steps 0 --replica-evals 0 --flips 0 --combined 0"
               " --surrogate 1 --sg-train 40 --sg-epochs 5 --sg-chain 20")
@@ -739,21 +793,24 @@ def main():
     ms_total = hip.elapsed_ms(e0, e1)
     evals_per_s = args.steps / (ms_total / 1e3)
 
-    # per-launch durations, events on the library stream around each: the
-    # gradient pass is one kernel (k_pass2; the mesh is closed, so no
-    # boundary walker follows it), the properties pass the launch sequence
-    # k_pass1_apart + k_rep_cached + k_finalize
+    # per-launch durations, events on the library stream around a run of
+    # back-to-back launches of one pass (a pair of events around every single
+    # launch would add the event's own cost to each): the gradient pass is
+    # one kernel (k_pass2; the mesh is closed, so no boundary walker follows
+    # it) and reads what the last properties pass left; the properties pass
+    # is the launch sequence k_pass1_apart + k_rep_cached + k_finalize
     ks = min(args.steps, 500)
-    evs = [[hip.event() for _ in range(3)] for _ in range(ks)]
-    for i in range(ks):
-        hip.record(evs[i][0], stream)
-        L.tm_energy_eval_device(em, mesh, capi.TM_EVAL_PROPS)
-        hip.record(evs[i][1], stream)
-        L.tm_energy_eval_device(em, mesh, capi.TM_EVAL_GRAD)
-        hip.record(evs[i][2], stream)
-    capi.check(L.tm_sync())
-    p1 = np.array([hip.elapsed_ms(e[0], e[1]) for e in evs])
-    p2 = np.array([hip.elapsed_ms(e[1], e[2]) for e in evs])
+
+    def per_launch_ms(flags, k):
+        hip.record(e0, stream)
+        for _ in range(k):
+            L.tm_energy_eval_device(em, mesh, flags)
+        hip.record(e1, stream)
+        capi.check(L.tm_sync())
+        return hip.elapsed_ms(e0, e1) / k
+
+    p1 = np.array([per_launch_ms(capi.TM_EVAL_PROPS, ks)])
+    p2 = np.array([per_launch_ms(capi.TM_EVAL_GRAD, ks)])
     capi.check(L.tm_energy_eval_device(em, mesh, both))
     capi.check(L.tm_sync())
 
@@ -845,7 +902,7 @@ def main():
     if args.flips:
         out["flips"] = flip_rates(v, f, params, args.precision)
     if args.combined:
-        out["combined"] = combined_rates(args.precision, hip, stream)
+        out["combined"] = combined_rates(args.precision, hip, stream, chain_steps=args.combined_chain)
     if not args.no_cpu_baseline:
         cv, ns, thr = cpu_baseline(v, f, params, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cv, "unit": "evals/s", "cores": thr, "kind": "port",

@@ -1582,7 +1582,7 @@ struct tm_replicas {
 
 namespace {
 
-void set_group_shape(tm_replicas& S, const Tuning& t);
+void set_group_shape(tm_replicas& S, const Tuning& t, bool force);
 
 // Rebuild the device table of pass arguments from the replicas' current
 // buffers and neighbour lists.  With a ladder, each replica's kappa_b is that
@@ -1637,11 +1637,14 @@ void refresh_replica_args(tm_replicas& S)
     h2d(S.d_args.p, S.h_args.data(), sizeof(PassArgs) * S.R);
     h2d(S.d_block0.p, S.h_block0.data(), sizeof(int) * (S.R + 1));
     h2d(S.d_block0_rc.p, b0c.data(), sizeof(int) * (S.R + 1));
-    set_group_shape(S, S.tune);    // raises every recording flag
+    set_group_shape(S, S.tune, true);
 }
 
-// block prefix tables of the repulsion query and pass 2 for tuning t
-void set_group_shape(tm_replicas& S, const Tuning& t)
+// block prefix tables of the repulsion query and pass 2 for tuning t; force:
+// every replica records at its next evaluation (new tables, or a pair cache
+// switched back on, which must not replay lists it did not keep).  The
+// recorded lists do not depend on the launch shape.
+void set_group_shape(tm_replicas& S, const Tuning& t, bool force)
 {
     std::vector<int> rep(S.R + 1, 0), p2(S.R + 1, 0);
     for (int r = 0; r < S.R; r++) {
@@ -1650,7 +1653,7 @@ void set_group_shape(tm_replicas& S, const Tuning& t)
     }
     S.tune = t;
     S.tune.rep_fused = 0;     // grouped launches keep the query apart
-    S.force_all();            // (a cache switched back on must not replay lists it did not keep)
+    if (force) S.force_all();
     S.total_rep = rep[S.R];
     S.total_p2 = p2[S.R];
     S.d_block0_rep.alloc(S.R + 1);
@@ -2089,7 +2092,8 @@ int tm_replicas_set_tuning(tm_replicas* rs, const tm_tuning* t)
     API_BEGIN
     require(t != nullptr && tuning_valid(from_c(*t)), "invalid launch shape");
     rs->drop_graph();
-    set_group_shape(*rs, from_c(*t));
+    const bool cache_on = from_c(*t).pair_cache && !rs->tune.pair_cache;
+    set_group_shape(*rs, from_c(*t), cache_on);
     API_END
 }
 
@@ -2113,14 +2117,14 @@ int tm_replicas_autotune(tm_replicas* rs, tm_tuning* out, double* ms_rep, double
     auto shaped = [&](const Tuning& t) {
         if (t.rep_threads != S.tune.rep_threads || t.rep_batch != S.tune.rep_batch ||
             t.p2_threads != S.tune.p2_threads)
-            set_group_shape(S, t);
+            set_group_shape(S, t, false);
         return S.group();
     };
     const Tuning best = search(
         S.tune, has_rep, [&](const Tuning& t) { launch_rep_grouped(shaped(t), g_rt.stream); },
         [&](const Tuning& t) { launch_pass2_grouped(shaped(t), g_rt.stream); }, ms_rep, ms_pass2);
     S.drop_graph();
-    set_group_shape(S, best);
+    set_group_shape(S, best, false);
     if (out) *out = to_c(best);
     API_END
 }
