@@ -542,8 +542,11 @@ def test_fd_gradient(core, bond):
     gradient of bindings.cpp:25-50, on the reference's test_gradient.py
     setup (icosphere(3), kappas 1, fractions 0.8, lc0 1.15 l, lc1 0.85 l,
     a0 = mean face area, epsilon 1e-8): within 1e-4 of the analytic gradient
-    (the reference test's bound) and within 1e-5 of the oracle's forward
-    differences, which round the same energies differently."""
+    (the reference test's bound) and, against the oracle's forward
+    differences, within 4x their rounding noise: each difference quotient
+    carries about 2 eps |E| / h of rounding, the two codes round their
+    energies independently (a factor sqrt 2), and the norm over n components
+    grows as sqrt n."""
     v, f = M.icosphere(3)
     om = O.Mesh(v, f)
     a, l = om.avg_tri_props()
@@ -556,10 +559,12 @@ def test_fd_gradient(core, bond):
     fd = np.empty((gm.n_vertices(), 3))
     core.gradient(gm, ge, fd, 1.0e-8)
     assert np.abs(gm.points() - v).max() <= 1e-15    # each probe is undone by subtracting epsilon
-    e_an, e_fd = rel(fd, ge.gradient(gm)), rel(fd, oe.fd_gradient(om, 1.0e-8))
-    print(f"bond {bond}: fd vs analytic {e_an:.2e}, fd vs oracle fd {e_fd:.2e}", flush=True)
+    g_or = oe.fd_gradient(om, 1.0e-8)
+    e_an, e_fd = rel(fd, ge.gradient(gm)), rel(fd, g_or)
+    noise = 2.0 * np.finfo(float).eps * abs(oe.energy(om)) / 1.0e-8 * np.sqrt(2.0 * fd.size) / np.linalg.norm(g_or)
+    print(f"bond {bond}: fd vs analytic {e_an:.2e}, fd vs oracle fd {e_fd:.2e} (noise {noise:.2e})", flush=True)
     assert e_an < 1e-4
-    assert e_fd < 1e-5
+    assert e_fd < 4.0 * noise
 
 
 def bipyramid(n=32, h=0.3):

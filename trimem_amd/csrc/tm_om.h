@@ -190,12 +190,19 @@ __device__ __forceinline__ double ipow(double x, int r)
 }
 
 // ------------------------------------------------- bond potentials --------
-// mesh_tether.h:40-55 (edge) and 94-102 (area), per halfedge
+// mesh_tether.h:40-55 (edge) and 94-102 (area), per halfedge.  Below lc1 the
+// reference divides by l - lc1, by l^r and (in the derivative) by (l - lc1)^2
+// and l; here one FP64 division gives w = 1 / (l (l - lc1)), from which
+// 1 / l = (l - lc1) w and 1 / (l - lc1) = l w, and the rest are products
+// (each a few ulp from the divided form).
 __device__ __forceinline__ double bond_edge_value(double l, const EParams& p)
 {
     int r = p.bond_r;
     if (l > p.bond_lc0) return ipow((double)r, r + 1) * ipow(l - p.bond_lc0, r);
-    if (l < p.bond_lc1) return exp(l / (l - p.bond_lc1)) / ipow(l, r);
+    if (l < p.bond_lc1) {
+        const double lmc = l - p.bond_lc1, w = 1.0 / (l * lmc);
+        return exp(l * (l * w)) * ipow(lmc * w, r);
+    }
     return 0.0;
 }
 // derivative factor dV/dl (mesh_tether.h:58-82; the two branches accumulate)
@@ -205,11 +212,11 @@ __device__ __forceinline__ double bond_edge_dfac(double l, const EParams& p)
     double fac = 0.0;
     if (l > p.bond_lc0) fac = ipow((double)r, r + 2) * ipow(l - p.bond_lc0, r - 1);
     if (l < p.bond_lc1) {
-        double lr = ipow(l, r);
-        double lmc = l - p.bond_lc1;
-        fac -= p.bond_lc1 / (lmc * lmc) + r / l;
-        fac /= lr;
-        fac *= exp(l / lmc);
+        const double lmc = l - p.bond_lc1, w = 1.0 / (l * lmc);
+        const double il = lmc * w, iq = l * w;    // 1 / l, 1 / (l - lc1)
+        fac -= p.bond_lc1 * (iq * iq) + r * il;
+        fac *= ipow(il, r);
+        fac *= exp(l * iq);
     }
     return fac;
 }
