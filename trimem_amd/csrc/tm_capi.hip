@@ -628,6 +628,7 @@ struct tm_energy {
     DevBuf<double> q, partials, grad, vprops, gprops;
     DevBuf<int> pair_cnt;
     DevBuf<double> rep_v, rep_g;
+    DevBuf<double> theta;    // pass 1's dihedral angles for pass 2 (kRingPad per vertex, float or double)
     DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<int> flip_result, proposals, pf_status;
@@ -681,6 +682,7 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
     E.grad.alloc((size_t)M.nv * 3);
     E.counter.alloc(2);
+    E.theta.alloc((size_t)kRingPad * M.nv);
     E.globals.alloc(1);
     E.flip_result.alloc(4);
     HIPCHECK(hipMemsetAsync(E.counter.p, 0, 2 * sizeof(unsigned), g_rt.stream));
@@ -733,6 +735,7 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.partials = E.partials.p;
     b.counter = E.counter.p;
     b.globals = E.globals.p;
+    b.theta = E.theta.p;
     b.list = M.generic.p;
     b.nlist = M.n_generic;
     tm_nlist& L = *E.nlist;

@@ -631,8 +631,8 @@ void launch_excl_mask(const MeshView& m, const NListView& nl, unsigned long long
 // Slot k's terms from its spoke e_k = x_{j_k} - x_v and the sector normals
 // n_k = e_k x e_{k-1} and n_{k+1}.
 template <class T>
-__device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e_cur, d3 n_cur, d3 n_next,
-                                                   double& area, double& vol, double& curv, double& teth)
+__device__ __forceinline__ T slot_properties_at(const EParams& p, d3 xv, d3 e_cur, d3 n_cur, d3 n_next,
+                                                double& area, double& vol, double& curv, double& teth)
 {
     T af = Prec<T>::sqrt_(dot_f(n_cur, n_cur)) * (T)0.5;
     const double l2 = dot_f(e_cur, e_cur);
@@ -643,6 +643,7 @@ __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e
     vol += dot_f(xv, n_cur);    // 6 x the signed volume x_v . (x_j x x_k) / 6 (mesh_util.h:94-101)
     if (p.bond_type == 0) teth += tether_value<T>(l2, l, p);
     else teth += bond_area_value((double)af, p);
+    return th;
 }
 
 // One lane per vertex.  Lane v walks the clockwise ring of v once, carrying
@@ -656,10 +657,13 @@ __device__ __forceinline__ void slot_properties_at(const EParams& p, d3 xv, d3 e
 // together.  Longer rings walk the table slot by slot.
 // use: v is an interior vertex (its sums are wanted).  The leading kRingPad
 // columns and their positions are loaded whether or not they are used, so
-// they need not wait for the meta word that decides it.
+// they need not wait for the meta word that decides it.  theta (optional):
+// the dihedral angle of each slot of a ring of at most kRingPad slots is
+// stored at theta[k * N + v] for pass 2, which then need not form it again.
 template <class T>
 __device__ __forceinline__ void ring_properties(const MeshView& m, const EParams& p, int v, bool use, int n, d3 xv,
-                                                double& area, double& vol, double& curv, double& teth)
+                                                double& area, double& vol, double& curv, double& teth,
+                                                T* __restrict__ theta)
 {
     const int N = m.nv;
     d3 e[kRingPad];
@@ -680,7 +684,8 @@ __device__ __forceinline__ void ring_properties(const MeshView& m, const EParams
             if (k < n) {
                 const d3 e_next = k + 1 < n ? e[(k + 1) % kRingPad] : e[0];
                 const d3 n_next = cross_f(e_next, e[k]);
-                slot_properties_at<T>(p, xv, e[k], n_cur, n_next, area, vol, curv, teth);
+                const T th = slot_properties_at<T>(p, xv, e[k], n_cur, n_next, area, vol, curv, teth);
+                if (theta) theta[k * N + v] = th;
                 n_cur = n_next;
             }
         }
@@ -752,7 +757,7 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     const bool ring = active && !is_generic(meta);
     double pr[6] = {0, 0, 0, 0, 0, 0};
     double area = 0, vol = 0, curv = 0, teth = 0;
-    ring_properties<T>(m, p, vc, ring, n, load_p(m.x, vc), area, vol, curv, teth);
+    ring_properties<T>(m, p, vc, ring, n, load_p(m.x, vc), area, vol, curv, teth, (T*)A.theta);
     if (ring) {
         // a third of the face areas and volumes, half the edge terms (by
         // the rounded reciprocals: at most the last bit differs from a
@@ -1321,7 +1326,7 @@ __global__ void __launch_bounds__(128) k_generic_grad_at(const PassArgs* __restr
 // straight into one accumulator.
 template <class T>
 __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& g, double qv, d3 e_prev, d3 e_cur,
-                                               double q_prev, double q_cur, d3 n_cur, d3 n_next)
+                                               double q_prev, double q_cur, d3 n_cur, d3 n_next, const T* th_given)
 {
     const double kb = p.kappa_b, kt = p.kappa_t;
     // face F_k = (v, j_k, j_{k-1})
@@ -1342,7 +1347,9 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
     // spoke (v, j_k) with faces F_k and F_{k+1}
     const double l2 = dot_f(e_cur, e_cur);
     T l = Prec<T>::sqrt_(l2);
-    T th = Prec<T>::dihedral(n_cur, n_next, e_cur, l);
+    // the angle pass 1 stored (the same operations on the same values), or
+    // formed here
+    const T th = th_given ? *th_given : Prec<T>::dihedral(n_cur, n_next, e_cur, l);
     double fe = -fma(kb, qv + q_cur, 0.5 * g.fac_c) * (double)th;
     if (p.bond_type == 0) fe = fma(-kt, tether_dfac<T>(l2, l, p), fe);
     return axpy_f(fe * (double)Prec<T>::rcp(l), e_cur, acc);
@@ -1357,21 +1364,24 @@ __device__ __forceinline__ d3 slot_gradient_at(const EParams& p, const Globals& 
 // the leading kRingPad ring columns of vc (they hold -1 past the valence,
 // which loads vc itself) and their positions and q before it knows whether
 // it needs them (use: an interior vertex with such a ring), so the loads do
-// not wait for the vertex's meta word.
-template <class T>
+// not wait for the vertex's meta word.  kStored: the slots' dihedral angles
+// are read from theta (pass 1 stored them), coalesced like the ring columns.
+template <class T, bool kStored>
 __device__ __forceinline__ d3 ring_gradient_short(const MeshView& m, const EParams& p, const Globals* __restrict__ gp,
                                                   const double* __restrict__ q, int vc, bool use, int n, d3 xv,
-                                                  double qv)
+                                                  double qv, const T* __restrict__ theta)
 {
     const int N = m.nv;
     d3 e[kRingPad];
     double qj[kRingPad];
+    T th[kRingPad];
 #pragma unroll
     for (int k = 0; k < kRingPad; k++) {
         const int j = m.ring[k * N + vc];
         const int jj = j >= 0 ? j : vc;
         e[k] = load_p(m.x, jj) - xv;
         qj[k] = q[jj];
+        if constexpr (kStored) th[k] = theta[k * N + vc];
     }
     d3 acc{0, 0, 0};
     if (!use) return acc;
@@ -1392,7 +1402,8 @@ __device__ __forceinline__ d3 ring_gradient_short(const MeshView& m, const EPara
             const d3 e_prev = k == 0 ? e_last : e[(k + kRingPad - 1) % kRingPad];
             const double q_prev = k == 0 ? q_last : qj[(k + kRingPad - 1) % kRingPad];
             const d3 n_next = cross_f(e_next, e[k]);
-            acc += slot_gradient_at<T>(p, g, qv, e_prev, e[k], q_prev, qj[k], n_cur, n_next);
+            acc += slot_gradient_at<T>(p, g, qv, e_prev, e[k], q_prev, qj[k], n_cur, n_next,
+                                       kStored ? &th[k] : nullptr);
             n_cur = n_next;
         }
     }
@@ -1423,7 +1434,7 @@ __device__ __forceinline__ d3 ring_gradient_walk(const MeshView& m, const EParam
             q_next = q[j];
         }
         const d3 n_next = cross_f(e_next, e_cur);
-        acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next);
+        acc += slot_gradient_at<T>(p, g, qv, e_prev, e_cur, q_prev, q_cur, n_cur, n_next, nullptr);
         e_prev = e_cur;
         q_prev = q_cur;
         e_cur = e_next;
@@ -1448,7 +1459,9 @@ __device__ __forceinline__ void pass2_body(const PassArgs& A, int lb)
     const double qv = A.q[vc];
     const int n = meta_valence(meta);
     const bool ring = active && !is_generic(meta);    // listed vertices: k_generic_grad
-    d3 acc = ring_gradient_short<T>(m, p, A.globals, A.q, vc, ring && n <= kRingPad, n, xv, qv);
+    const bool use = ring && n <= kRingPad;
+    d3 acc = A.theta ? ring_gradient_short<T, true>(m, p, A.globals, A.q, vc, use, n, xv, qv, (const T*)A.theta)
+                     : ring_gradient_short<T, false>(m, p, A.globals, A.q, vc, use, n, xv, qv, nullptr);
     if (ring && n > kRingPad) acc = ring_gradient_walk<T>(m, p, *A.globals, A.q, v, n, xv, qv);
     // repulsion gradient of the vertices the query found pairs for (written
     // by the query kernel of this evaluation)
@@ -1512,6 +1525,7 @@ PassArgs pass_args(const MeshView& m, const NListView& nl, const EParams& p, con
     a.nlist = b.nlist;
     a.nblocks = pass1_blocks(m.nv);
     a.pc = a.rep_v ? b.pc : PairCache{};
+    a.theta = b.theta;
     return a;
 }
 

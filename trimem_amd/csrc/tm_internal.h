@@ -54,6 +54,10 @@ struct PassBuffers {
     const int* list;    // boundary / degenerate vertices
     int nlist;
     PairCache pc;       // x_snap null: the query runs in full every time
+    // [kRingPad * N] dihedral angle of each ring slot of vertices of valence
+    // <= kRingPad, column-major, in the precision of the pass (float or
+    // double); written by pass 1, read by pass 2.  Null: pass 2 forms them
+    void* theta;
 };
 int pass1_blocks(int nv);
 
@@ -75,6 +79,7 @@ struct PassArgs {
     Globals* globals;
     double* grad;
     double* vprops;      // optional per-vertex properties of pass 1
+    void* theta;         // PassBuffers::theta
     const int* list;     // boundary / degenerate vertices
     int nlist;
     int nblocks;         // pass-1 blocks of kPass1Verts vertices (a multiple of 8)
