@@ -372,9 +372,9 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
         R.hmc_run(1, swap_every=1 if (k + 1) % swap_every == 0 else 0)
         capi.check(L.tm_sync())
         t1 = time.perf_counter()
-        for i, (m, e) in enumerate(zip(ms, es)):
-            flips_acc += core.pflip(m, e, flip_ratio, seed=1000 * k + i)
-            flips_tried += int(m.n_edges() * flip_ratio)
+        # one parallel sweep of every replica, round by round over all of them
+        flips_acc += sum(R.pflip(flip_ratio, [1000 * k + i for i in range(n_replicas)]))
+        flips_tried += sum(int(m.n_edges() * flip_ratio) for m in ms)
         capi.check(L.tm_sync())
         t_hmc += t1 - t0
         t_flip += time.perf_counter() - t1
@@ -387,8 +387,8 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
                     "swap_acceptance": float(st["swaps_accepted"].sum() / max(st["swaps_tried"].sum(), 1)),
                     "flip_acceptance": flips_acc / max(flips_tried, 1),
                     "setup": f"HMC dt {time_step}, 10 leapfrog steps, T 1, device draws; parallel flip sweep of "
-                             f"{flip_ratio:.0%} of the edges of every replica after every HMC step; one kappa_b "
-                             f"exchange round every {swap_every} steps"}
+                             f"{flip_ratio:.0%} of the edges of every replica (all replicas in one sweep) after "
+                             f"every HMC step; one kappa_b exchange round every {swap_every} steps"}
 
     # the surrogate's fallback on replica 0
     sg = core.Surrogate(ms[0], es[0], seed=3)

@@ -282,6 +282,13 @@ int tm_replicas_destroy(tm_replicas* rs);
 /* TM_EVAL_PROPS | TM_EVAL_GRAD for every replica; results in each energy
  * manager's device buffers (tm_energy_device_globals / _gradient) */
 int tm_replicas_eval(tm_replicas* rs, int flags);
+/* pflip (flip_parallel_batches, flips.cpp:82-166) on every replica at once:
+ * replica r draws its proposals from seeds[r] and flips with the kappa_b of
+ * the ladder slot it holds; each replica's flips and accepted count are
+ * those of tm_flip_parallel(ems[r], meshes[r], flip_ratio, seeds[r], 0, .)
+ * (round k of every replica runs in one launch per phase).  accepted:
+ * int[n_replicas] or NULL.  flip_ratio > 1 -> TM_E_RANGE. */
+int tm_replicas_pflip(tm_replicas* rs, double flip_ratio, const uint64_t* seeds, int* accepted);
 /* out[7 r + k]: the six global properties (area, volume, curvature,
  * bending, tethering, repulsion) and the energy of the last evaluation of
  * replica r (synchronises) */

@@ -300,3 +300,30 @@ def test_member_rebuild_refreshes_group(core):
     for i, (m, e) in enumerate(reps):
         R.eval()
         assert R.globals()[i, 6] == e.energy(m), i
+
+
+def test_grouped_pflip_equals_single(core):
+    """Replicas.pflip: every replica's accepted flips, faces and the energy
+    and gradient after them equal pflip on that mesh alone with the same
+    seed (meshes of different sizes, so different batch sizes and rounds,
+    both list types, an open tube), over two sweeps; and a grouped
+    evaluation after the sweeps sees the new topologies."""
+    seeds = [101, 202, 303, 404, 505]
+    a = mixed_set(core, "mixed")
+    b = mixed_set(core, "mixed")
+    R = core.Replicas([m for m, _ in a], [e for _, e in a])
+    R.eval()
+    for sweep in range(2):
+        got = R.pflip(0.2, [s + sweep for s in seeds])
+        want = [core.pflip(m, e, 0.2, seed=s + sweep) for (m, e), s in zip(b, seeds)]
+        assert got == want
+        assert sum(got) > 0
+        for (ma, ea), (mb, eb) in zip(a, b):
+            assert np.array_equal(ma.fv_indices(), mb.fv_indices())
+            assert ea.energy(ma) == eb.energy(mb)
+    R.eval()
+    g = R.globals()
+    for i, ((ma, ea), (mb, eb)) in enumerate(zip(a, b)):
+        assert g[i, 6] == eb.energy(mb)
+    with pytest.raises(Exception):
+        R.pflip(1.5, seeds)

@@ -824,6 +824,16 @@ PYBIND11_MODULE(core, m)
                 check(tm_replicas_eval(r.h, (props ? TM_EVAL_PROPS : 0) | (grad ? TM_EVAL_GRAD : 0)));
             },
             py::arg("properties") = true, py::arg("gradient") = true)
+        .def(
+            "pflip",
+            [](Replicas& r, double ratio, const std::vector<uint64_t>& seeds) {
+                if ((int)seeds.size() != r.size()) throw std::runtime_error("one seed per replica");
+                std::vector<int> acc(r.size(), 0);
+                check(tm_replicas_pflip(r.h, ratio, seeds.data(), acc.data()));
+                return acc;
+            },
+            "pflip on every replica at once (replica r draws from seeds[r]); accepted flips per replica",
+            py::arg("flip_ratio"), py::arg("seeds"))
         .def("globals",
              [](Replicas& r) {
                  py::array_t<double> out({(size_t)r.size(), (size_t)7});

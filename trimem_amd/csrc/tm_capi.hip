@@ -1540,6 +1540,10 @@ struct tm_replicas {
     // displacement check and of the cached query
     DevBuf<int> force_flags, d_block0_rc;
     DevBuf<double> disp_parts;
+    // parallel flip sweeps of the group: per-replica arguments and block
+    // prefix tables (rebuilt by every sweep)
+    DevBuf<PflipArgs> pf_args;
+    DevBuf<int> pf_claim0, pf_eval0, pf_apply0;
     std::vector<int> h_ones;
     int total_disp = 0, total_rc = 0;    // displacement partials, cached-query blocks
     std::vector<unsigned> gens;    // each member's tm_energy::gen when the table was built
@@ -1747,6 +1751,81 @@ int tm_replicas_destroy(tm_replicas* rs)
 {
     delete rs;
     return TM_OK;
+}
+
+int tm_replicas_pflip(tm_replicas* rs, double flip_ratio, const uint64_t* seeds, int* accepted)
+{
+    API_BEGIN
+    require(seeds != nullptr, "null seeds");
+    // flips.cpp:84-85
+    if (flip_ratio > 1.0) throw TmError(TM_E_RANGE, "flip_ratio must be <= 1.0");
+    tm_replicas& S = *rs;
+    // each replica's kappa_b is that of the ladder slot it holds now
+    std::vector<int> slot_of(S.R);
+    if (S.has_ladder) {
+        std::vector<int> owner(S.R);
+        d2h(owner.data(), S.owner.p, sizeof(int) * S.R);
+        sync();
+        for (int s = 0; s < S.R; s++) slot_of[owner[s]] = s;
+    }
+    std::vector<PflipArgs> h(S.R);
+    std::vector<int> c0(S.R + 1, 0), e0(S.R + 1, 0), a0(S.R + 1, 0);
+    int rounds = 0;
+    for (int r = 0; r < S.R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        tm_energy& E = *S.ems[r];
+        // as tm_flip_parallel: the FP64 properties at the current positions
+        // start each replica's Metropolis walk
+        upload_points(M);
+        run_pass1(E, M, 0, nullptr);
+        const int ne = M.nh / 2, nflips = (int)(ne * flip_ratio), batch = std::max(1, ne / 1024);
+        if (!E.pf_owner.p || E.pf_owner.n < (size_t)ne) E.pf_owner.alloc(ne);
+        HIPCHECK(hipMemsetAsync(E.pf_owner.p, 0, sizeof(unsigned long long) * ne, g_rt.stream));
+        E.pf_status.alloc(batch);
+        E.pf_dprops.alloc((size_t)12 * batch);
+        E.pf_state.alloc(8);
+        EParams ep = eparams_of(E);
+        if (S.has_ladder) ep.kappa_b = S.h_ladder[slot_of[r]];
+        h[r] = PflipArgs{M.flip_view(), E.nlist->view(), ep, seeds[r], nflips, batch, E.globals.p, E.pf_owner.p,
+                         E.pf_status.p, E.pf_dprops.p, E.pf_dprops.p + 6 * (size_t)batch, E.pf_state.p};
+        if (nflips > 0) rounds = std::max(rounds, (nflips + batch - 1) / batch);
+        c0[r + 1] = c0[r] + (batch + 15) / 16;
+        e0[r + 1] = e0[r] + batch;
+        a0[r + 1] = a0[r] + (batch + 127) / 128;
+    }
+    S.pf_args.alloc(S.R);
+    S.pf_claim0.alloc(S.R + 1);
+    S.pf_eval0.alloc(S.R + 1);
+    S.pf_apply0.alloc(S.R + 1);
+    h2d(S.pf_args.p, h.data(), sizeof(PflipArgs) * S.R);
+    h2d(S.pf_claim0.p, c0.data(), sizeof(int) * (S.R + 1));
+    h2d(S.pf_eval0.p, e0.data(), sizeof(int) * (S.R + 1));
+    h2d(S.pf_apply0.p, a0.data(), sizeof(int) * (S.R + 1));
+    PflipGroup g{S.pf_args.p, S.R, rounds, S.ems[0]->tune.pflip_window, S.pf_claim0.p, c0[S.R],
+                 S.pf_eval0.p, e0[S.R], S.pf_apply0.p, a0[S.R]};
+    launch_flip_parallel_grouped(g, g_rt.stream);
+    check_launch();
+    std::vector<double> st(8 * (size_t)S.R, 0.0);
+    std::vector<int> ovf(S.R, 0);
+    for (int r = 0; r < S.R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        HIPCHECK(hipMemsetAsync(M.flag.p, 0, sizeof(int), g_rt.stream));
+        launch_build_ring(M.view(), M.ring.p, M.meta.p, M.flag.p, g_rt.stream);
+        check_launch();
+        if (h[r].nflips > 0) d2h(&st[8 * (size_t)r], S.ems[r]->pf_state.p, 8 * sizeof(double));
+        d2h(&ovf[r], M.flag.p, sizeof(int));
+    }
+    sync();
+    bool overflow = false;
+    for (int r = 0; r < S.R; r++) {
+        tm_mesh& M = *S.meshes[r];
+        M.host_stale = true;
+        if (st[8 * (size_t)r + 7] > 0 || ovf[r]) M.topo_version++;
+        overflow = overflow || ovf[r];
+        if (accepted) accepted[r] = (int)st[8 * (size_t)r + 7];
+    }
+    if (overflow) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
+    API_END
 }
 
 int tm_replicas_eval(tm_replicas* rs, int flags)

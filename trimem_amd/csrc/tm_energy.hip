@@ -708,17 +708,6 @@ __device__ __forceinline__ bool is_generic(int meta)
     return meta_bslot(meta) >= 0 || meta_valence(meta) < 3;
 }
 
-// replica of logical block lb in a grouped launch (block0 ascending, block0[0] = 0)
-__device__ __forceinline__ int find_replica(const int* __restrict__ block0, int R, int lb)
-{
-    int lo = 0, hi = R - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (block0[mid] <= lb) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Pass 1 over the kPass1Verts vertices of block lb; vertices on the boundary
 // (or with fewer than three neighbours) read the properties that

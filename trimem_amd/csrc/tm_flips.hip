@@ -1138,10 +1138,10 @@ __device__ __forceinline__ unsigned long long pflip_key(int round, int B, int i)
 
 // four lanes per proposal, one per patch vertex (the maximum is the same in
 // any order of the atomics)
-__global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
-                                                    unsigned long long* __restrict__ owner)
+__device__ __forceinline__ void pflip_claim_body(const FlipMeshView& fm, uint64_t seed, int round, int B, int nprop,
+                                                 unsigned long long* __restrict__ owner, int lb)
 {
-    int i = blockIdx.x * 16 + threadIdx.x / 4;
+    int i = lb * 16 + threadIdx.x / 4;
     if (i >= nprop) return;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
     int e = pflip_edge(seed, round, B, i, fm.nh / 2);
@@ -1150,20 +1150,26 @@ __global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t se
     for_patch_edges_of(t, e, threadIdx.x & 3, [&](int pe) { atomicMax(&owner[pe], key); });
 }
 
+__global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
+                                                    unsigned long long* __restrict__ owner)
+{
+    pflip_claim_body(fm, seed, round, B, nprop, owner, blockIdx.x);
+}
+
 // status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected.
 // One workgroup per proposal and side: lanes 0-3 check that the proposal's
 // key survived on their part of the patch while lane 4 tests is_flip_ok; a
 // winner's workgroups then evaluate the patch before the flip (side 0,
 // dold = -old) and after it on FlippedTopo (side 1, dnew = new).
-__global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
-                                                   int round, int B, int nprop,
-                                                   const unsigned long long* __restrict__ owner,
-                                                   int* __restrict__ status, double* __restrict__ dold,
-                                                   double* __restrict__ dnew)
+__device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NListView& nl, const EParams& p,
+                                                uint64_t seed, int round, int B, int nprop,
+                                                const unsigned long long* __restrict__ owner,
+                                                int* __restrict__ status, double* __restrict__ dold,
+                                                double* __restrict__ dnew, int i)
 {
     __shared__ CoopShared S;
     __shared__ int s_go;
-    const int i = blockIdx.x, side = blockIdx.y;
+    const int side = blockIdx.y;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
     const int e = pflip_edge(seed, round, B, i, fm.nh / 2);
     if (threadIdx.x == 0) {
@@ -1196,6 +1202,15 @@ __global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl
     }
 }
 
+__global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
+                                                   int round, int B, int nprop,
+                                                   const unsigned long long* __restrict__ owner,
+                                                   int* __restrict__ status, double* __restrict__ dold,
+                                                   double* __restrict__ dnew)
+{
+    pflip_eval_body(fm, nl, p, seed, round, B, nprop, owner, status, dold, dnew, blockIdx.x);
+}
+
 __global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
                               const int* __restrict__ status, int want)
 {
@@ -1217,11 +1232,9 @@ __global__ void k_pflip_apply(FlipMeshView fm, uint64_t seed, int round, int B, 
 constexpr int kScanThreads = 256;
 constexpr int kScanChunk = 1024;
 
-__global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop,
-                                                             int window, int* __restrict__ status,
-                                                             const double* __restrict__ dold,
-                                                             const double* __restrict__ dnew,
-                                                             double* __restrict__ state)
+__device__ __forceinline__ void pflip_scan_body(const EParams& p, uint64_t seed, int round, int B, int nprop,
+                                                int window, int* __restrict__ status, const double* __restrict__ dold,
+                                                const double* __restrict__ dnew, double* __restrict__ state)
 {
     __shared__ int s_st[kScanChunk];
     __shared__ double s_dp[6 * kScanChunk];
@@ -1279,13 +1292,79 @@ __global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t
     if (tid < 8) state[tid] = s_state[tid];
 }
 
-__global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* __restrict__ state)
+__global__ void __launch_bounds__(kScanThreads) k_pflip_scan(EParams p, uint64_t seed, int round, int B, int nprop,
+                                                             int window, int* __restrict__ status,
+                                                             const double* __restrict__ dold,
+                                                             const double* __restrict__ dnew,
+                                                             double* __restrict__ state)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    pflip_scan_body(p, seed, round, B, nprop, window, status, dold, dnew, state);
+}
+
+__device__ __forceinline__ void pflip_init_body(const Globals* __restrict__ g0, const EParams& p,
+                                                double* __restrict__ state)
+{
     double props[6];
     for (int k = 0; k < 6; k++) props[k] = state[k] = g0->props[k];
     state[6] = energy_of(props, p);
     state[7] = 0.0;
+}
+
+__global__ void k_pflip_init(const Globals* __restrict__ g0, EParams p, double* __restrict__ state)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    pflip_init_body(g0, p, state);
+}
+
+// ---- many meshes: round r of every replica per launch ----
+// proposals of replica a in round r (<= 0 once its sweep is done)
+__device__ __forceinline__ int pflip_nprop(const PflipArgs& a, int round)
+{
+    return min(a.batch, a.nflips - round * a.batch);
+}
+
+__global__ void k_pflip_init_grouped(const PflipArgs* __restrict__ A)
+{
+    if (threadIdx.x == 0) pflip_init_body(A[blockIdx.x].g0, A[blockIdx.x].p, A[blockIdx.x].state);
+}
+
+__global__ void __launch_bounds__(64) k_pflip_claim_grouped(const PflipArgs* __restrict__ A,
+                                                            const int* __restrict__ block0, int R, int round)
+{
+    const int r = find_replica(block0, R, blockIdx.x);
+    const PflipArgs& a = A[r];
+    const int nprop = pflip_nprop(a, round);
+    if (nprop <= 0) return;
+    pflip_claim_body(a.fm, a.seed, round, a.batch, nprop, a.owner, blockIdx.x - block0[r]);
+}
+
+__global__ void __launch_bounds__(64) k_pflip_eval_grouped(const PflipArgs* __restrict__ A,
+                                                           const int* __restrict__ block0, int R, int round)
+{
+    const int r = find_replica(block0, R, blockIdx.x);
+    const PflipArgs& a = A[r];
+    const int nprop = pflip_nprop(a, round), i = blockIdx.x - block0[r];
+    if (i >= nprop) return;
+    pflip_eval_body(a.fm, a.nl, a.p, a.seed, round, a.batch, nprop, a.owner, a.status, a.dold, a.dnew, i);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_pflip_scan_grouped(const PflipArgs* __restrict__ A, int round,
+                                                                     int window)
+{
+    const PflipArgs& a = A[blockIdx.x];
+    const int nprop = pflip_nprop(a, round);
+    if (nprop <= 0) return;
+    pflip_scan_body(a.p, a.seed, round, a.batch, nprop, window, a.status, a.dold, a.dnew, a.state);
+}
+
+__global__ void k_pflip_apply_grouped(const PflipArgs* __restrict__ A, const int* __restrict__ block0, int R,
+                                      int round)
+{
+    const int r = find_replica(block0, R, blockIdx.x);
+    const PflipArgs& a = A[r];
+    const int i = (blockIdx.x - block0[r]) * blockDim.x + threadIdx.x;
+    if (i >= pflip_nprop(a, round) || a.status[i] != 2) return;
+    do_flip(a.fm, pflip_edge(a.seed, round, a.batch, i, a.fm.nh / 2));
 }
 
 void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
@@ -1304,6 +1383,19 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
                            std::min(std::max(window, 1), kScanThreads), status, dprops, dprops + 6 * (size_t)batch,
                            state);
         hipLaunchKernelGGL(k_pflip_apply, grid, block, 0, s, fm, seed, r, batch, nprop, status, 2);
+    }
+}
+
+void launch_flip_parallel_grouped(const PflipGroup& g, hipStream_t s)
+{
+    if (g.R <= 0) return;
+    hipLaunchKernelGGL(k_pflip_init_grouped, dim3(g.R), dim3(64), 0, s, g.d_args);
+    const int window = std::min(std::max(g.window, 1), kScanThreads);
+    for (int r = 0; r < g.rounds; r++) {
+        hipLaunchKernelGGL(k_pflip_claim_grouped, dim3(g.total_claim), dim3(64), 0, s, g.d_args, g.d_claim0, g.R, r);
+        hipLaunchKernelGGL(k_pflip_eval_grouped, dim3(g.total_eval, 2), dim3(64), 0, s, g.d_args, g.d_eval0, g.R, r);
+        hipLaunchKernelGGL(k_pflip_scan_grouped, dim3(g.R), dim3(kScanThreads), 0, s, g.d_args, r, window);
+        hipLaunchKernelGGL(k_pflip_apply_grouped, dim3(g.total_apply), dim3(128), 0, s, g.d_args, g.d_apply0, g.R, r);
     }
 }
 

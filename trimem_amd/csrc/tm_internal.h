@@ -224,6 +224,52 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
                           int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status,
                           double* dprops, double* state, hipStream_t s);
 
+// The parallel sweep of many meshes at once: round r of every replica in one
+// launch per phase.  Each replica's kernels run the single-mesh sweep's
+// device functions on its own arguments, so its flips, decisions and state
+// are those of launch_flip_parallel with the same arguments.
+struct PflipArgs {
+    FlipMeshView fm;
+    NListView nl;
+    EParams p;
+    uint64_t seed;
+    int nflips, batch;
+    const Globals* g0;
+    unsigned long long* owner;
+    int* status;
+    double* dold;    // [6 batch]
+    double* dnew;    // [6 batch]
+    double* state;   // [8]
+};
+struct PflipGroup {
+    const PflipArgs* d_args;
+    int R;
+    int rounds;            // the most rounds of any replica
+    int window;
+    // prefix tables of blocks per replica: claims (16 proposals per block),
+    // evaluations (one proposal per block), applications (128 per block)
+    const int* d_claim0;
+    int total_claim;
+    const int* d_eval0;
+    int total_eval;
+    const int* d_apply0;
+    int total_apply;
+};
+void launch_flip_parallel_grouped(const PflipGroup& g, hipStream_t s);
+
+// replica of logical block lb in a grouped launch (block0 ascending,
+// block0[0] = 0)
+__device__ __forceinline__ int find_replica(const int* __restrict__ block0, int R, int lb)
+{
+    int lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (block0[mid] <= lb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // device HMC step (tm_hmc.hip)
 struct HmcState {
     double energy0, energy1, kinetic0, kinetic1, dh, uniform;
