@@ -264,6 +264,12 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
 constexpr int kCoopLanes = 16;
 constexpr int kCoopChunk = 4 * kCoopLanes;
 
+// LDS capacity of the one-ring of each ring vertex (exclusion level 2); a
+// patch with a ring vertex of larger valence takes the exact om_excluded
+// walk instead (the same decisions, from global memory).  Smaller than the
+// ring-table width so that more workgroups fit a CU's LDS.
+constexpr int kRing2Cap = 16;
+
 struct CoopShared {
     int h[4][kMaxValence];              // outgoing halfedges, circulator order
     int nb[4][kMaxValence];             // their target vertices (the one-ring)
@@ -272,7 +278,7 @@ struct CoopShared {
     int total[4];                       // repulsion candidates
     int slow[4];                        // exclusion tests by om_excluded
     double slot[4][kMaxValence][4];     // area, volume, edge curvature, tether
-    int ring2[4][kMaxValence][kMaxValence];
+    int ring2[4][kMaxValence][kRing2Cap];
     int ring2_n[4][kMaxValence];
     double rep[4][kCoopChunk];
     double vp[4][6];
@@ -405,7 +411,7 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
             if (g0 >= 0) {
                 int gh = g0;
                 do {
-                    if (m == kMaxValence) {
+                    if (m == kRing2Cap) {
                         S.slow[g] = 1;
                         break;
                     }
