@@ -264,10 +264,12 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
 constexpr int kCoopLanes = 16;
 constexpr int kCoopChunk = 4 * kCoopLanes;
 
-// LDS capacity of the one-ring of each ring vertex (exclusion level 2); a
-// patch with a ring vertex of larger valence takes the exact om_excluded
-// walk instead (the same decisions, from global memory).  Smaller than the
-// ring-table width so that more workgroups fit a CU's LDS.
+// LDS capacity of the exclusion-2 neighbourhood: the one-rings of up to 16
+// ring vertices of each patch vertex, up to 16 vertices each.  A patch
+// vertex of larger valence, or with a ring vertex of larger valence, takes
+// the exact om_excluded walk instead (the same decisions, from global
+// memory).  Smaller than the ring-table width so that more workgroups fit a
+// CU's LDS.
 constexpr int kRing2Cap = 16;
 
 struct CoopShared {
@@ -278,8 +280,8 @@ struct CoopShared {
     int total[4];                       // repulsion candidates
     int slow[4];                        // exclusion tests by om_excluded
     double slot[4][kMaxValence][4];     // area, volume, edge curvature, tether
-    int ring2[4][kMaxValence][kRing2Cap];
-    int ring2_n[4][kMaxValence];
+    int ring2[4][kRing2Cap][kRing2Cap];
+    int ring2_n[4][kRing2Cap];
     double rep[4][kCoopChunk];
     double vp[4][6];
     double sum[6];
@@ -405,7 +407,9 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
             S.slot[g][s][2] = 0.5 * el * da;
             S.slot[g][s][3] = p.bond_type == 0 ? bond_edge_value(el, p) : bond_area_value(mu_face_area(t, h), p);
         }
-        if (rep_on && excl > 1) {
+        if (rep_on && excl > 1 && s >= kRing2Cap) {
+            S.slow[g] = 1;
+        } else if (rep_on && excl > 1) {
             const int g0 = t.vhe(S.nb[g][s]);
             int m = 0;
             if (g0 >= 0) {
