@@ -174,34 +174,13 @@ __device__ __forceinline__ void block_sum6(double v[6], double* s_red /* [nwaves
     __syncthreads();
 }
 
-// The six globals from the block partials, then the energy (kernel.h:110-122)
-// and the penalty factors 2 kappa / ref (X / ref - 1) (kernel.h:27-71).  One
-// wave computes them: lane i sums rows i, i + 64, ... of each column in that
-// order (loading a sweep of rows before adding it, so their round trips
-// overlap), then a butterfly joins the 64 lanes.  The order depends on
-// nothing but the number of rows, so every kernel that calls this gets the
-// same bits; every lane of the wave returns the result.  The partials are
-// stored column by column (partials[k * nblocks + b]).
-__device__ __forceinline__ Globals reduce_globals(const double* __restrict__ partials, int nblocks, const EParams& p)
+__device__ __forceinline__ Globals globals_of_sums(double v[6], const EParams& p);
+
+// The end of finalize_block: the butterfly over the wave's lanes, then the
+// energy (kernel.h:110-122) and the penalty factors 2 kappa / ref (X / ref -
+// 1) (kernel.h:27-71).
+__device__ __forceinline__ Globals globals_of_sums(double v[6], const EParams& p)
 {
-    constexpr int kRows = 8;
-    const int lane = threadIdx.x & 63;
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int b0 = lane; b0 < nblocks; b0 += kRows * 64) {
-        double r[kRows][6];
-#pragma unroll
-        for (int u = 0; u < kRows; u++) {
-            const int b = b0 + u * 64;
-#pragma unroll
-            for (int k = 0; k < 6; k++) r[u][k] = b < nblocks ? partials[k * nblocks + b] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kRows; u++)
-            if (b0 + u * 64 < nblocks) {
-#pragma unroll
-                for (int k = 0; k < 6; k++) v[k] += r[u][k];
-            }
-    }
 #pragma unroll
     for (int k = 0; k < 6; k++)
 #pragma unroll
@@ -1225,17 +1204,58 @@ __global__ void __launch_bounds__(kThreads) k_rep_grouped(const PassArgs* __rest
 
 // Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
 // factors (one wave; kernel boundaries order it after pass 1).
-__global__ void __launch_bounds__(64) k_finalize(PassArgs A)
+// The six globals from the block partials (stored column by column,
+// partials[k * nblocks + b]) in a fixed order: lane i of wave 0 adds rows
+// i, i + 64, ... of each column in turn, then a butterfly joins the 64
+// lanes.  The order depends on nothing but the number of rows, so every
+// launch that calls this gets the same bits.  The whole block stages up to
+// kFinRows rows in LDS with one round of loads (17 values per thread in
+// flight) before wave 0 adds them; kFinRows is a multiple of 64, so chunks
+// keep each lane's rows in order.  Valid in wave 0.
+constexpr int kFinThreads = 256;
+constexpr int kFinRows = 704;
+__device__ __forceinline__ Globals finalize_block(const double* __restrict__ partials, int nblocks, const EParams& p)
 {
-    const Globals g = reduce_globals(A.partials, A.nblocks, A.p);
+    static_assert(kFinRows % 64 == 0, "chunks of whole lane sweeps");
+    constexpr int kBatch = (6 * kFinRows + kFinThreads - 1) / kFinThreads;
+    __shared__ double s_rows[6 * kFinRows];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int c0 = 0; c0 < nblocks; c0 += kFinRows) {
+        const int rows = min(kFinRows, nblocks - c0);
+        double r[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int i = threadIdx.x + u * kFinThreads, k = i / kFinRows, b = i - k * kFinRows;
+            r[u] = i < 6 * kFinRows && b < rows ? partials[k * nblocks + c0 + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int i = threadIdx.x + u * kFinThreads;
+            if (i < 6 * kFinRows) s_rows[i] = r[u];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64)
+            for (int b = threadIdx.x; b < rows; b += 64)
+#pragma unroll
+                for (int k = 0; k < 6; k++) v[k] += s_rows[k * kFinRows + b];
+        __syncthreads();
+    }
+    return globals_of_sums(v, p);
+}
+
+// Fixed-order FP64 sum of the block partials of pass 1, energy and penalty
+// factors (kernel boundaries order it after pass 1).
+__global__ void __launch_bounds__(kFinThreads) k_finalize(PassArgs A)
+{
+    const Globals g = finalize_block(A.partials, A.nblocks, A.p);
     if (threadIdx.x == 0) *A.globals = g;
 }
 
-// one wave per replica
-__global__ void __launch_bounds__(64) k_finalize_grouped(const PassArgs* __restrict__ A)
+// one block per replica
+__global__ void __launch_bounds__(kFinThreads) k_finalize_grouped(const PassArgs* __restrict__ A)
 {
     const PassArgs& a = A[blockIdx.x];
-    const Globals g = reduce_globals(a.partials, a.nblocks, a.p);
+    const Globals g = finalize_block(a.partials, a.nblocks, a.p);
     if (threadIdx.x == 0) {
         *a.globals = g;
         if (a.pc.dforce) *a.pc.dforce = 0;    // the query of this evaluation recorded
@@ -1643,7 +1663,7 @@ void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int 
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
     launch_pass1_body(a, mixed, t, s);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, a);
 }
 
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
@@ -1691,7 +1711,7 @@ void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
         if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
         else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
     }
-    hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(64), 0, s, g.d_args);
+    hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(kFinThreads), 0, s, g.d_args);
 }
 
 void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s)
