@@ -1,14 +1,16 @@
 """GPU: the surrogate force network (trimem_amd.core.Surrogate, tm_surrogate_*).
 
 Pinned by:
-  * forward: the GPU's predicted forces against the CPU restatement
-    (tests/surrogate_ref.py, bf16-rounded matrix operands like the MFMA
-    kernels, features rebuilt from the oracle's properties), relative error
-    <= 1e-2 over the whole force array;
-  * backward: the GPU parameter gradient of the loss against torch autograd
-    of the same restatement: cosine >= 0.99 for every weight tensor, relative
-    error <= 5e-2 overall (the restatement itself is pinned by finite
-    differences in tests/test_surrogate_ref.py);
+  * forward: every hidden stage and the predicted forces against the CPU
+    restatement (tests/surrogate_ref.py, bf16-rounded matrix operands like
+    the MFMA kernels, features rebuilt from the oracle's properties):
+    stages within 2e-5 and forces within 4e-5 relative (measured 1.5e-6 and
+    3.8e-6);
+  * backward: the GPU loss within 1e-5 and its parameter gradient against
+    torch autograd of the same restatement: cosine >= 0.99999 for every
+    weight tensor, relative error <= 5e-3 overall (measured 0.9999984 and
+    8.6e-4; the restatement itself is pinned by finite differences in
+    tests/test_surrogate_ref.py);
   * training on GPU-recorded snapshots lowers the loss, on the snapshots and
     on a held-out state;
   * exact acceptance: a surrogate-driven HMC step reports the exact energies
@@ -104,10 +106,13 @@ def test_forward_matches_reference(core):
             a = sg.activations(which)
             e = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
             print("level", level, "stage", which, "rel err", e, flush=True)
+            # measured <= 1.5e-6 (FP32 accumulation of bf16 products in a
+            # different order): a bound 10x above it
+            assert e <= 2e-5, (level, which)
         Fr = S.forward(torch.tensor(np.asarray(sg.params)), fe, sp, nu, sF).numpy()
         err = np.linalg.norm(F - Fr) / np.linalg.norm(Fr)
         print("level", level, "forward rel err", err, flush=True)
-        assert err <= 1e-2
+        assert err <= 4e-5    # measured <= 3.8e-6
 
 
 def test_backward_matches_autograd(core):
@@ -122,7 +127,7 @@ def test_backward_matches_autograd(core):
     (g_ref,) = torch.autograd.grad(L, th)
     g_ref = g_ref.numpy()
     print("loss gpu", loss_gpu, "ref", float(L), flush=True)
-    assert abs(loss_gpu - float(L)) <= 2e-2 * float(L)
+    assert abs(loss_gpu - float(L)) <= 1e-5 * float(L)    # measured 9e-7
     off = S.OFF
     blocks = [("Win", off["Win"], off["bin"]), ("bin", off["bin"], off["U0"])]
     for l in range(S.LAYERS):
@@ -136,10 +141,10 @@ def test_backward_matches_autograd(core):
             continue
         cos = float(x @ y / (np.linalg.norm(x) * np.linalg.norm(y)))
         print(name, "cos", cos, flush=True)
-        assert cos >= 0.99, name
+        assert cos >= 0.99999, name    # measured >= 0.9999984
     rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
     print("gradient rel err", rel, flush=True)
-    assert rel <= 5e-2
+    assert rel <= 5e-3    # measured 8.6e-4
 
 
 def test_training_lowers_loss(core):
