@@ -418,7 +418,9 @@ def ess(x):
     n = len(x)
     x = x - x.mean()
     var = float(x @ x) / n
-    if n < 4 or var == 0.0:
+    if var == 0.0:
+        return 1.0    # a chain that never moved holds one sample
+    if n < 4:
         return float(n)
     f = np.fft.rfft(x, 2 * n)
     acf = np.fft.irfft(f * np.conj(f))[:n] / (n * var)
