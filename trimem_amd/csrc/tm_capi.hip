@@ -629,7 +629,6 @@ struct tm_energy {
     DevBuf<int> pair_cnt;
     DevBuf<double> rep_v, rep_g;
     DevBuf<double> theta;    // pass 1's dihedral angles for pass 2 (kRingPad per vertex, float or double)
-    DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<int> flip_result, proposals, pf_status;
     DevBuf<double> uniforms, pf_dprops, pf_state;
@@ -681,11 +680,9 @@ void ensure_energy_buffers(tm_energy& E, const tm_mesh& M)
     E.rep_g.alloc((size_t)M.nv * 3);
     E.partials.alloc((size_t)pass1_blocks(M.nv) * 6);
     E.grad.alloc((size_t)M.nv * 3);
-    E.counter.alloc(2);
     E.theta.alloc((size_t)kRingPad * M.nv);
     E.globals.alloc(1);
     E.flip_result.alloc(4);
-    HIPCHECK(hipMemsetAsync(E.counter.p, 0, 2 * sizeof(unsigned), g_rt.stream));
 }
 
 // update_repulsion (energy.cpp:57-61) without new device allocations: the
@@ -733,7 +730,6 @@ PassBuffers buffers(tm_energy& E, tm_mesh& M)
     b.rep_v = E.rep_v.p;
     b.rep_g = E.rep_g.p;
     b.partials = E.partials.p;
-    b.counter = E.counter.p;
     b.globals = E.globals.p;
     b.theta = E.theta.p;
     b.list = M.generic.p;
@@ -2231,7 +2227,6 @@ struct tm_surrogate {
     // the feature pass: properties without repulsion, in buffers of its own
     DevBuf<double> q, gprops, vprops, partials, rep_v;
     DevBuf<int> pair_cnt;
-    DevBuf<unsigned> counter;
     DevBuf<Globals> globals;
     DevBuf<double> neg_force, xsave;
     DevBuf<float> losses;
@@ -2303,7 +2298,6 @@ void sg_alloc(tm_surrogate& S)
     S.partials.alloc((size_t)pass1_blocks(S.nv) * 6);
     S.rep_v.alloc(N);
     S.pair_cnt.alloc(N);
-    S.counter.alloc(2);
     S.globals.alloc(1);
     S.neg_force.alloc(3 * N);
     S.xsave.alloc(3 * N);
@@ -2326,7 +2320,6 @@ void sg_feature_pass(tm_surrogate& S)
     pb.pair_cnt = S.pair_cnt.p;
     pb.rep_v = S.rep_v.p;
     pb.partials = S.partials.p;
-    pb.counter = S.counter.p;
     pb.globals = S.globals.p;
     pb.list = M.generic.p;
     pb.nlist = 0;

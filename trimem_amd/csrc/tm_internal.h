@@ -49,7 +49,6 @@ struct PassBuffers {
     double* rep_v;      // [N] repulsion property per vertex (k_rep)
     double* rep_g;      // [3N] repulsion gradient, rows of vertices with pairs (k_rep)
     double* partials;   // [6 * pass1 blocks]
-    unsigned* counter;  // last-block counter of pass 1 (returns to 0)
     Globals* globals;
     const int* list;    // boundary / degenerate vertices
     int nlist;
