@@ -327,3 +327,24 @@ def test_grouped_pflip_equals_single(core):
         assert g[i, 6] == eb.energy(mb)
     with pytest.raises(Exception):
         R.pflip(1.5, seeds)
+
+
+def test_grouped_pflip_uses_the_ladder(core):
+    """With a kappa_b ladder, each replica flips with the kappa_b of the slot
+    it holds: the same accepted flips and faces as pflip on a mesh whose
+    own kappa_b is that slot's value (the group's meshes were created with
+    kappa_b 1, so ignoring the ladder would decide differently)."""
+    ladder = [5.0, 20.0, 60.0, 150.0]
+    seeds = [71, 72, 73, 74]
+    a = [membrane(core, 3, 40 + i, "fp64", kappa_b=1.0) for i in range(4)]
+    b = [membrane(core, 3, 40 + i, "fp64", kappa_b=ladder[i]) for i in range(4)]
+    R = core.Replicas([m for m, _ in a], [e for _, e in a])
+    R.hmc_setup(mass=1.0, time_step=1e-4, temperature=1.0, seeds=[1, 2, 3, 4], num_integration_steps=2)
+    R.set_ladder(ladder, seed=5)
+    slot_of = R.stats()["slot_of"]
+    assert list(slot_of) == [0, 1, 2, 3]
+    got = R.pflip(0.5, seeds)
+    want = [core.pflip(m, e, 0.5, seed=s) for (m, e), s in zip(b, seeds)]
+    assert got == want
+    for (ma, _), (mb, _) in zip(a, b):
+        assert np.array_equal(ma.fv_indices(), mb.fv_indices())
