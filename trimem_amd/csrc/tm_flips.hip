@@ -251,26 +251,16 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
 //   phase 1  lane 0 of the group walks the one-ring (clockwise from v_he,
 //            as the circulator does); every lane locates the candidate runs
 //            of the repulsion query;
-//   phase 2  the lanes compute the per-halfedge terms and, for exclusion
-//            level 2, gather the one-rings of the ring vertices into LDS;
+//   phase 2  the lanes compute the per-halfedge terms;
 //   phase 3  the lanes test the candidates in chunks of 64, the exclusion
-//            test reading LDS instead of walking halfedges per candidate;
+//            test reading the one-ring from LDS (coop_excluded);
 //   phase 4  lane 0 adds the terms of each accumulator in the order the
 //            single-thread walker (om_vertex_properties) adds them, so every
 //            value is bit-identical to it; thread 0 adds the four vertices.
 // A vertex whose valence exceeds kMaxValence is evaluated by its lane 0
-// with om_vertex_properties; a ring vertex whose valence exceeds it sends
-// that group's exclusion tests to om_excluded.
+// with om_vertex_properties.
 constexpr int kCoopLanes = 16;
 constexpr int kCoopChunk = 4 * kCoopLanes;
-
-// LDS capacity of the exclusion-2 neighbourhood: the one-rings of up to 16
-// ring vertices of each patch vertex, up to 16 vertices each.  A patch
-// vertex of larger valence, or with a ring vertex of larger valence, takes
-// the exact om_excluded walk instead (the same decisions, from global
-// memory).  Smaller than the ring-table width so that more workgroups fit a
-// CU's LDS.
-constexpr int kRing2Cap = 16;
 
 struct CoopShared {
     int h[4][kMaxValence];              // outgoing halfedges, circulator order
@@ -278,10 +268,7 @@ struct CoopShared {
     int bnd[4][kMaxValence];            // halfedge on the boundary
     int n[4];                           // valence; -1 above kMaxValence
     int total[4];                       // repulsion candidates
-    int slow[4];                        // exclusion tests by om_excluded
     double slot[4][kMaxValence][4];     // area, volume, edge curvature, tether
-    int ring2[4][kRing2Cap][kRing2Cap];
-    int ring2_n[4][kRing2Cap];
     double rep[4][kCoopChunk];
     double vp[4][6];
     double sum[6];
@@ -344,17 +331,35 @@ __device__ __forceinline__ int cand_slot(const CandRuns& r, int c)
     return q;
 }
 
-__device__ __forceinline__ bool coop_excluded(const CoopShared& S, int g, int n, int excl, int j)
+__device__ __forceinline__ bool coop_in_ring(const CoopShared& S, int g, int n, int j)
+{
+    for (int k = 0; k < n; k++)
+        if (S.nb[g][k] == j) return true;
+    return false;
+}
+
+// om_excluded(ve, j) for patch vertex g (valence n <= kMaxValence, one-ring
+// in S.nb).  Adjacency is symmetric, so j lies in the one-ring of a ring
+// vertex of ve exactly when j's own one-ring meets ve's: the walk runs
+// around j, and only for candidates within the cutoff that are not in ve's
+// one-ring, instead of around every ring vertex of every patch vertex.  A
+// candidate of valence above kMaxValence takes om_excluded.
+template <class TP>
+__device__ __forceinline__ bool coop_excluded(const TP& t, const CoopShared& S, int g, int n, int ve, int excl,
+                                              int j)
 {
     if (excl <= 0) return false;
-    for (int k = 0; k < n; k++) {
-        if (S.nb[g][k] == j) return true;
-        if (excl > 1) {
-            const int m = S.ring2_n[g][k];
-            for (int q = 0; q < m; q++)
-                if (S.ring2[g][k][q] == j) return true;
-        }
-    }
+    if (coop_in_ring(S, g, n, j)) return true;
+    if (excl == 1) return false;
+    const int h0 = t.vhe(j);
+    if (h0 < 0) return false;
+    int h = h0, m = 0;
+    do {
+        if (m == kMaxValence) return om_excluded(t, ve, j, excl);
+        if (coop_in_ring(S, g, n, t.to(h))) return true;
+        m++;
+        h = t.cw(h);
+    } while (h != h0);
     return false;
 }
 
@@ -389,7 +394,6 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
         }
         S.n[g] = n;
         S.total[g] = (rep_on && n >= 0) ? runs.total : 0;
-        S.slow[g] = 0;
     }
     __syncthreads();
     const int n = S.n[g];
@@ -407,31 +411,12 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
             S.slot[g][s][2] = 0.5 * el * da;
             S.slot[g][s][3] = p.bond_type == 0 ? bond_edge_value(el, p) : bond_area_value(mu_face_area(t, h), p);
         }
-        if (rep_on && excl > 1 && s >= kRing2Cap) {
-            S.slow[g] = 1;
-        } else if (rep_on && excl > 1) {
-            const int g0 = t.vhe(S.nb[g][s]);
-            int m = 0;
-            if (g0 >= 0) {
-                int gh = g0;
-                do {
-                    if (m == kRing2Cap) {
-                        S.slow[g] = 1;
-                        break;
-                    }
-                    S.ring2[g][s][m++] = t.to(gh);
-                    gh = t.cw(gh);
-                } while (gh != g0);
-            }
-            S.ring2_n[g][s] = m;
-        }
     }
     __syncthreads();
 
     // phase 3
     const int total = S.total[g];
     const int tmax = max(max(S.total[0], S.total[1]), max(S.total[2], S.total[3]));
-    const bool slow = S.slow[g] != 0;
     const double dmax = p.rep_lc1;
     const double dmax2 = dmax * dmax * (1.0 + 1e-15);    // prefilter, as in nlist_visit
     double rep = 0.0;
@@ -448,7 +433,7 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
                 const double sq = sqnorm(d);
                 bool in = sq <= dmax2 && sqrt(sq) < dmax;
                 if (in && nl.kind == 0)
-                    in = j != ve && !(slow ? om_excluded(t, ve, j, excl) : coop_excluded(S, g, n, excl, j));
+                    in = j != ve && !coop_excluded(t, S, g, n, ve, excl, j);
                 if (in) val[k] = rep_value(norm(d), p.rep_lc1, p.rep_r);
             }
             S.rep[g][k * kCoopLanes + l] = val[k];
