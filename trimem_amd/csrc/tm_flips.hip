@@ -249,8 +249,8 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
 // to(next(2e+1)), by one 64-lane workgroup: lanes 16g .. 16g+15 evaluate
 // patch vertex g.
 //   phase 1  lane 0 of the group walks the one-ring (clockwise from v_he,
-//            as the circulator does); every lane locates the candidate runs
-//            of the repulsion query;
+//            as the circulator does) and locates the candidate runs of the
+//            repulsion query;
 //   phase 2  the lanes compute the per-halfedge terms;
 //   phase 3  the lanes test the candidates in chunks of 64, the exclusion
 //            test reading the one-ring from LDS (coop_excluded);
@@ -260,6 +260,12 @@ __device__ __forceinline__ double energy_of(const double pr[6], const EParams& p
 // A vertex whose valence exceeds kMaxValence is evaluated by its lane 0
 // with om_vertex_properties.
 constexpr int kCoopLanes = 16;
+// The parallel sweep's patch kernels keep to 128 registers (four waves per
+// SIMD, 16 workgroups per CU): a sweep over a replica group has far more
+// patches than the GPU holds at once, and occupancy, not one patch's
+// latency, then bounds it (at 153 registers, 12 workgroups per CU, a round
+// over 32 meshes of 491520 edges took 21% longer).
+#define kCoopOccupancy __attribute__((amdgpu_waves_per_eu(4)))
 constexpr int kCoopChunk = 4 * kCoopLanes;
 
 struct CoopShared {
@@ -269,6 +275,10 @@ struct CoopShared {
     int n[4];                           // valence; -1 above kMaxValence
     int total[4];                       // repulsion candidates
     double slot[4][kMaxValence][4];     // area, volume, edge curvature, tether
+    int run_s[4][9], run_len[4][9];     // candidate runs (CandRuns) of the repulsion query
+    double pos[4][3];                   // the patch vertex's position
+    int ve[4];                          // the patch vertices
+    int flip_ok;                        // flip_ok(e), when coop_patch tests it
     double rep[4][kCoopChunk];
     double vp[4][6];
     double sum[6];
@@ -364,19 +374,32 @@ __device__ __forceinline__ bool coop_excluded(const TP& t, const CoopShared& S, 
 }
 
 // Called by all 64 threads of the workgroup; S.vp and S.sum hold the result
-// on return.
+// on return.  test_flip (on the topology before the flip, e interior): S.flip_ok
+// = flip_ok(t, e) from the one-rings of phase 1, which include those of the
+// two vertices opposite the edge (patch vertices 1 and 3), and nothing more
+// is evaluated when it is false.
 template <class TP>
-__device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, int e, CoopShared& S)
+__device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, int e, CoopShared& S,
+                           bool test_flip = false)
 {
     const int g = threadIdx.x / kCoopLanes, l = threadIdx.x % kCoopLanes;
     const int ve = patch_vertex(t, e, g);
     const bool rep_on = p.rep_lc1 > 0 && nl.kind >= 0;
     const int excl = nl.kind == 0 ? nl.exclusion : 0;
-    const d3 pi = t.P(ve);
 
-    // phase 1
-    CandRuns runs = cand_runs(nl, ve, pi);
+    // phase 1 (the candidate runs wait in LDS while phase 2 needs the registers)
     if (l == 0) {
+        const d3 pv = t.P(ve);
+        S.pos[g][0] = pv.x;
+        S.pos[g][1] = pv.y;
+        S.pos[g][2] = pv.z;
+        const CandRuns runs = cand_runs(nl, ve, pv);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            S.run_s[g][k] = runs.s[k];
+            S.run_len[g][k] = runs.len[k];
+        }
+        S.total[g] = rep_on ? runs.total : 0;
         int n = 0;
         int h0 = t.vhe(ve);
         if (h0 >= 0) {
@@ -393,9 +416,22 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
             } while (h != h0);
         }
         S.n[g] = n;
-        S.total[g] = (rep_on && n >= 0) ? runs.total : 0;
+        S.ve[g] = ve;
+        if (n < 0) S.total[g] = 0;
     }
     __syncthreads();
+    if (test_flip) {
+        // flip_ok's walks: the opposite vertices differ, are not adjacent
+        // and have valence below kMaxValence (n is -1 above it)
+        if (threadIdx.x == 0) {
+            const int a = S.ve[1], b = S.ve[3], na = S.n[1], nb = S.n[3];
+            bool ok = a != b && na >= 0 && na < kMaxValence && nb >= 0 && nb < kMaxValence;
+            for (int k = 0; ok && k < na; k++) ok = S.nb[1][k] != b;
+            S.flip_ok = ok;
+        }
+        __syncthreads();
+        if (!S.flip_ok) return;
+    }
     const int n = S.n[g];
 
     // phase 2
@@ -415,6 +451,13 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
     __syncthreads();
 
     // phase 3
+    const d3 pi{S.pos[g][0], S.pos[g][1], S.pos[g][2]};
+    CandRuns runs;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        runs.s[k] = S.run_s[g][k];
+        runs.len[k] = S.run_len[g][k];
+    }
     const int total = S.total[g];
     const int tmax = max(max(S.total[0], S.total[1]), max(S.total[2], S.total[3]));
     const double dmax = p.rep_lc1;
@@ -671,14 +714,21 @@ __global__ void __launch_bounds__(64) k_sf_eval(FlipMeshView fm, NListView nl, E
     if (side == 0 && tid < 4)
         w.pv[4 * k + tid] = t.edge_boundary(e) ? t.to(2 * e + (tid & 1)) : patch_vertex(t, e, tid);
     if (side == 0 && tid == 0) w.e[k] = e;
-    if (tid == 0) s_ok = flip_ok(t, e);
+    if (tid == 0) s_ok = !t.edge_boundary(e);
     __syncthreads();
     if (!s_ok) {
         if (side == 0 && tid == 0) w.valid[k] = 0;
         return;
     }
+    // side 0 decides flip_ok from its one-rings; side 1 evaluates the
+    // flipped patch without waiting for it (FlippedTopo is a valid view of
+    // any interior edge; its values are read only for valid proposals)
     if (side == 0) {
-        coop_patch(t, nl, p, e, S);
+        coop_patch(t, nl, p, e, S, true);
+        if (!S.flip_ok) {
+            if (tid == 0) w.valid[k] = 0;
+            return;
+        }
         const unsigned long long key = sf_key(st->window, W, k);
         if (tid == 0) {
             for (int c = 0; c < 6; c++) w.dO[6 * k + c] = S.sum[c];
@@ -1153,8 +1203,8 @@ __global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t se
 
 // status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected.
 // One workgroup per proposal and side: lanes 0-3 check that the proposal's
-// key survived on their part of the patch while lane 4 tests is_flip_ok; a
-// winner's workgroups then evaluate the patch before the flip (side 0,
+// key survived on their part of the patch; a winner's workgroups then
+// evaluate the patch before the flip (side 0, which also tests is_flip_ok,
 // dold = -old) and after it on FlippedTopo (side 1, dnew = new).
 __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NListView& nl, const EParams& p,
                                                 uint64_t seed, int round, int B, int nprop,
@@ -1178,13 +1228,15 @@ __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NL
         bool win = true;
         for_patch_edges_of(t, e, threadIdx.x, [&](int pe) { win = win && owner[pe] == key; });
         if (!win) s_go = 0;
-    } else if (threadIdx.x == 4) {
-        if (!flip_ok(t, e)) s_go = 0;
     }
     __syncthreads();
     if (!s_go) return;
+    // side 0 decides flip_ok from its one-rings; side 1 evaluates the
+    // flipped patch without waiting for it (FlippedTopo is a valid view of
+    // any interior edge; dnew is read only for winners that pass flip_ok)
     if (side == 0) {
-        coop_patch(t, nl, p, e, S);
+        coop_patch(t, nl, p, e, S, true);
+        if (!S.flip_ok) return;
         if (threadIdx.x == 0) {
             for (int k = 0; k < 6; k++) dold[6 * i + k] = -S.sum[k];
             status[i] = 1;
@@ -1197,7 +1249,7 @@ __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NL
     }
 }
 
-__global__ void __launch_bounds__(64) k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
+__global__ void __launch_bounds__(64) kCoopOccupancy k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
                                                    int round, int B, int nprop,
                                                    const unsigned long long* __restrict__ owner,
                                                    int* __restrict__ status, double* __restrict__ dold,
@@ -1333,7 +1385,7 @@ __global__ void __launch_bounds__(64) k_pflip_claim_grouped(const PflipArgs* __r
     pflip_claim_body(a.fm, a.seed, round, a.batch, nprop, a.owner, blockIdx.x - block0[r]);
 }
 
-__global__ void __launch_bounds__(64) k_pflip_eval_grouped(const PflipArgs* __restrict__ A,
+__global__ void __launch_bounds__(64) kCoopOccupancy k_pflip_eval_grouped(const PflipArgs* __restrict__ A,
                                                            const int* __restrict__ block0, int R, int round)
 {
     const int r = find_replica(block0, R, blockIdx.x);
