@@ -279,6 +279,7 @@ struct CoopShared {
     double pos[4][3];                   // the patch vertex's position
     int ve[4];                          // the patch vertices
     int flip_ok;                        // flip_ok(e), when coop_patch tests it
+    int lost[4];                        // a patch edge of the vertex is owned by another proposal
     double rep[4][kCoopChunk];
     double vp[4][6];
     double sum[6];
@@ -377,10 +378,15 @@ __device__ __forceinline__ bool coop_excluded(const TP& t, const CoopShared& S, 
 // on return.  test_flip (on the topology before the flip, e interior): S.flip_ok
 // = flip_ok(t, e) from the one-rings of phase 1, which include those of the
 // two vertices opposite the edge (patch vertices 1 and 3), and nothing more
-// is evaluated when it is false.
+// is evaluated when it is false.  owner (same topology): S.lost = some edge
+// of the parallel sweep's patch, the spokes h >> 1 and rims next(h) >> 1 of
+// every outgoing halfedge h of the four vertices (for_patch_edges_of), is
+// owned by another key, tested on phase 2's halfedges; nothing more is
+// evaluated then.
 template <class TP>
 __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, int e, CoopShared& S,
-                           bool test_flip = false)
+                           bool test_flip = false, const unsigned long long* __restrict__ owner = nullptr,
+                           unsigned long long key = 0)
 {
     const int g = threadIdx.x / kCoopLanes, l = threadIdx.x % kCoopLanes;
     const int ve = patch_vertex(t, e, g);
@@ -418,6 +424,16 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
         S.n[g] = n;
         S.ve[g] = ve;
         if (n < 0) S.total[g] = 0;
+        // a ring too long for LDS: its patch edges by a walk of their own
+        bool lost = false;
+        if (owner && n < 0 && h0 >= 0) {
+            int h = h0, guard = 0;
+            do {
+                lost = lost || owner[h >> 1] != key || owner[t.next(h) >> 1] != key;
+                h = t.cw(h);
+            } while (h != h0 && ++guard < 4 * kMaxValence);
+        }
+        S.lost[g] = lost;
     }
     __syncthreads();
     if (test_flip) {
@@ -447,8 +463,10 @@ __device__ void coop_patch(const TP& t, const NListView& nl, const EParams& p, i
             S.slot[g][s][2] = 0.5 * el * da;
             S.slot[g][s][3] = p.bond_type == 0 ? bond_edge_value(el, p) : bond_area_value(mu_face_area(t, h), p);
         }
+        if (owner && (owner[h >> 1] != key || owner[t.next(h) >> 1] != key)) S.lost[g] = 1;
     }
     __syncthreads();
+    if (owner && (S.lost[0] | S.lost[1] | S.lost[2] | S.lost[3])) return;
 
     // phase 3
     const d3 pi{S.pos[g][0], S.pos[g][1], S.pos[g][2]};
@@ -1202,10 +1220,14 @@ __global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t se
 }
 
 // status: 0 skipped, 1 winner pending, 2 accepted, 3 rejected.
-// One workgroup per proposal and side: lanes 0-3 check that the proposal's
-// key survived on their part of the patch; a winner's workgroups then
-// evaluate the patch before the flip (side 0, which also tests is_flip_ok,
-// dold = -old) and after it on FlippedTopo (side 1, dnew = new).
+// One workgroup per proposal and side.  Side 0 evaluates the patch before
+// the flip (dold = -old) and decides: the proposal's key survived on every
+// patch edge and is_flip_ok holds, both read off the one-rings coop_patch
+// walks anyway.  Side 1 evaluates the patch after the flip on FlippedTopo
+// (dnew = new) without waiting for the decision: FlippedTopo is a valid view
+// of any interior edge, every walk on it is bounded, and dnew is read only
+// for the proposals side 0 marks pending (a loser's evaluation is wasted,
+// a few percent of them at E / 1024 proposals per round).
 __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NListView& nl, const EParams& p,
                                                 uint64_t seed, int round, int B, int nprop,
                                                 const unsigned long long* __restrict__ owner,
@@ -1213,30 +1235,14 @@ __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NL
                                                 double* __restrict__ dnew, int i)
 {
     __shared__ CoopShared S;
-    __shared__ int s_go;
     const int side = blockIdx.y;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
     const int e = pflip_edge(seed, round, B, i, fm.nh / 2);
-    if (threadIdx.x == 0) {
-        if (side == 0) status[i] = 0;
-        s_go = 1;
-    }
+    if (threadIdx.x == 0 && side == 0) status[i] = 0;
     if (t.edge_boundary(e)) return;
-    __syncthreads();
-    const unsigned long long key = pflip_key(round, B, i);
-    if (threadIdx.x < 4) {
-        bool win = true;
-        for_patch_edges_of(t, e, threadIdx.x, [&](int pe) { win = win && owner[pe] == key; });
-        if (!win) s_go = 0;
-    }
-    __syncthreads();
-    if (!s_go) return;
-    // side 0 decides flip_ok from its one-rings; side 1 evaluates the
-    // flipped patch without waiting for it (FlippedTopo is a valid view of
-    // any interior edge; dnew is read only for winners that pass flip_ok)
     if (side == 0) {
-        coop_patch(t, nl, p, e, S, true);
-        if (!S.flip_ok) return;
+        coop_patch(t, nl, p, e, S, true, owner, pflip_key(round, B, i));
+        if (!S.flip_ok || (S.lost[0] | S.lost[1] | S.lost[2] | S.lost[3])) return;
         if (threadIdx.x == 0) {
             for (int k = 0; k < 6; k++) dold[6 * i + k] = -S.sum[k];
             status[i] = 1;
