@@ -2233,7 +2233,7 @@ struct tm_surrogate {
     std::vector<std::unique_ptr<DevBuf<double>>> snap_x, snap_g;
     int adam_step = 0;
     // plain-gradient fallback of tm_hmc_step_guarded
-    int pol_window = 10;         // surrogate-driven outcomes judged together
+    int pol_window = 10;         // surrogate-driven outcomes counted together
     double pol_min_acc = 0.5;    // below this acceptance the exact gradient drives
     int pol_probe = 10;          // while it does: one surrogate-driven probe every pol_probe steps
     std::deque<int> recent;      // outcomes (1 accepted) of the last surrogate-driven steps
