@@ -1785,7 +1785,7 @@ int tm_replicas_pflip(tm_replicas* rs, double flip_ratio, const uint64_t* seeds,
         h[r] = PflipArgs{M.flip_view(), E.nlist->view(), ep, seeds[r], nflips, batch, E.globals.p, E.pf_owner.p,
                          E.pf_status.p, E.pf_dprops.p, E.pf_dprops.p + 6 * (size_t)batch, E.pf_state.p};
         if (nflips > 0) rounds = std::max(rounds, (nflips + batch - 1) / batch);
-        c0[r + 1] = c0[r] + (batch + 15) / 16;
+        c0[r + 1] = c0[r] + (batch + kClaimPerBlock - 1) / kClaimPerBlock;
         e0[r + 1] = e0[r] + batch;
         a0[r + 1] = a0[r] + (batch + 127) / 128;
     }

@@ -1199,18 +1199,41 @@ __device__ __forceinline__ unsigned long long pflip_key(int round, int B, int i)
     return ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
 }
 
-// four lanes per proposal, one per patch vertex (the maximum is the same in
-// any order of the atomics)
+// Eight lanes per proposal, two per patch vertex: for_patch_edges_of's walk
+// around the vertex, lane 0 of the pair clockwise from v_he and lane 1
+// counter-clockwise (the inverse of cw, prev(h) ^ 1), each claiming its
+// halfedges' spokes and rims until they meet, so the chain of dependent
+// loads is half as long.  The same edges get the same atomicMax, and the
+// maximum does not depend on the order of the atomics.
 __device__ __forceinline__ void pflip_claim_body(const FlipMeshView& fm, uint64_t seed, int round, int B, int nprop,
                                                  unsigned long long* __restrict__ owner, int lb)
 {
-    int i = lb * 16 + threadIdx.x / 4;
+    const int i = lb * kClaimPerBlock + threadIdx.x / 8;
     if (i >= nprop) return;
     const Topo t{fm.x, fm.he_to, fm.he_next, fm.he_prev, fm.he_face, fm.v_he, fm.f_he};
-    int e = pflip_edge(seed, round, B, i, fm.nh / 2);
-    if (t.edge_boundary(e)) return;
+    const int e = pflip_edge(seed, round, B, i, fm.nh / 2);
+    if (t.edge_boundary(e)) return;    // both lanes of every pair leave together
     const unsigned long long key = pflip_key(round, B, i);
-    for_patch_edges_of(t, e, threadIdx.x & 3, [&](int pe) { atomicMax(&owner[pe], key); });
+    const int w = (threadIdx.x / 2) & 3, back = threadIdx.x & 1;
+    const int h0 = t.vhe(patch_vertex(t, e, w));
+    if (h0 < 0) return;
+    // a: the clockwise walker's next halfedge, b: the counter-clockwise
+    // walker's; both lanes hold both, so they leave the loop together
+    int a = h0, b = t.prev(h0) ^ 1, pb = -1;
+#pragma unroll 1
+    for (int it = 0; it < 2 * kMaxValence && a != pb; it++) {
+        const int h = back ? b : a;
+        if (!back || a != b) {
+            atomicMax(&owner[h >> 1], key);
+            atomicMax(&owner[t.next(h) >> 1], key);
+        }
+        if (a == b) break;
+        const int nxt = back ? (t.prev(b) ^ 1) : t.cw(a);
+        const int oth = __shfl_xor(nxt, 1, 64);
+        pb = b;
+        a = back ? oth : nxt;
+        b = back ? nxt : oth;
+    }
 }
 
 __global__ void __launch_bounds__(64) k_pflip_claim(FlipMeshView fm, uint64_t seed, int round, int B, int nprop,
@@ -1429,7 +1452,8 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
     for (int r = 0; r < rounds; r++) {
         int nprop = std::min(batch, nflips - r * batch);
         dim3 grid((nprop + 127) / 128), block(128);
-        hipLaunchKernelGGL(k_pflip_claim, dim3((nprop + 15) / 16), dim3(64), 0, s, fm, seed, r, batch, nprop, owner);
+        hipLaunchKernelGGL(k_pflip_claim, dim3((nprop + kClaimPerBlock - 1) / kClaimPerBlock), dim3(64), 0, s, fm, seed, r,
+                           batch, nprop, owner);
         hipLaunchKernelGGL(k_pflip_eval, dim3(nprop, 2), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, owner,
                            status, dprops, dprops + 6 * (size_t)batch);
         hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop,

@@ -223,6 +223,10 @@ void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EPa
                           int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status,
                           double* dprops, double* state, hipStream_t s);
 
+// Proposals per 64-lane block of the parallel sweep's claim phase (two lanes
+// per patch vertex).
+constexpr int kClaimPerBlock = 8;
+
 // The parallel sweep of many meshes at once: round r of every replica in one
 // launch per phase.  Each replica's kernels run the single-mesh sweep's
 // device functions on its own arguments, so its flips, decisions and state
