@@ -564,20 +564,102 @@ __device__ __forceinline__ bool cached_neighbours(const MeshView& m, const NList
 // the i-th candidate of the list of v's build-time cell is excluded from v
 // (excluded_fast, which depends on the ring tables only).  8-lane groups,
 // lane l takes candidates l, l + 8, ...; the group ORs its words.
+// The group first gathers v's excluded set, v, its ring and (exclusion 2)
+// the rings of its ring vertices, into an LDS hash set (open addressing,
+// linear probing, -1 empty), one ring vertex per lane; each candidate then
+// costs a probe or two instead of reading its own ring.  Membership is exact
+// (adjacency is symmetric, so this is the set excluded_fast tests).  A ring
+// above kGroup, or a set above kMaskSetMax entries, takes excluded_fast.
+constexpr int kMaskTab = 128;
+constexpr int kMaskSetMax = 96;
+
+__device__ __forceinline__ unsigned mask_hash(int j)
+{
+    return ((unsigned)j * 2654435761u) >> 25;    // 7 bits: kMaskTab == 128
+}
+
+__device__ __forceinline__ bool mask_insert(int* tab, int* cnt, int j)
+{
+    const unsigned h = mask_hash(j);
+    for (int q = 0; q < kMaskTab; q++) {
+        const int at = (h + q) & (kMaskTab - 1);
+        const int old = atomicCAS(&tab[at], -1, j);
+        if (old == -1) return atomicAdd(cnt, 1) < kMaskSetMax;
+        if (old == j) return true;
+    }
+    return false;
+}
+
+// a set of at most kMaskSetMax entries: the probe ends at j or at an empty slot
+__device__ __forceinline__ bool mask_member(const int* tab, int j)
+{
+    const unsigned h = mask_hash(j);
+    for (int q = 0; q < kMaskTab; q++) {
+        const int x = tab[(h + q) & (kMaskTab - 1)];
+        if (x == j) return true;
+        if (x < 0) return false;
+    }
+    return false;
+}
+
 __global__ void __launch_bounds__(256) k_excl_mask(MeshView m, NListView nl, unsigned long long* __restrict__ mask)
 {
+    constexpr int kGroups = 256 / kGroup;
+    __shared__ int s_tab[kGroups][kMaskTab];
+    __shared__ int s_cnt[kGroups];
+    __shared__ int s_ovf[kGroups];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = threadIdx.x / kGroup;
     const int v = t / kGroup, lane = t % kGroup;
     const bool active = v < m.nv;
+    const int excl = nl.exclusion;
+    for (int i = lane; i < kMaskTab; i += kGroup) s_tab[gi][i] = -1;
+    if (lane == 0) {
+        s_cnt[gi] = 0;
+        s_ovf[gi] = 0;
+    }
+    int n = 0;
+    RingRegs R;
+    R.n = 0;
+    if (active) {
+        n = meta_valence(m.meta[v]);
+        R = load_ring(m, v, n);
+    }
+    const bool use_set = active && excl >= 1 && n <= kGroup;
+    __syncthreads();
+    if (use_set) {
+        bool ok = lane != 0 || mask_insert(s_tab[gi], &s_cnt[gi], v);
+        int w = -1;
+#pragma unroll
+        for (int k = 0; k < kGroup; k++)
+            if (k == lane) w = R.r[k];
+        if (w >= 0) {
+            ok = mask_insert(s_tab[gi], &s_cnt[gi], w) && ok;
+            if (excl > 1) {
+                const int N = m.nv;
+                int c[kGroup];
+#pragma unroll
+                for (int a = 0; a < kGroup; a++) c[a] = m.ring[a * N + w];
+#pragma unroll
+                for (int a = 0; a < kGroup; a++)
+                    if (c[a] >= 0) ok = mask_insert(s_tab[gi], &s_cnt[gi], c[a]) && ok;
+                if (c[kGroup - 1] >= 0) {
+                    const int nw = meta_valence(m.meta[w]);
+                    for (int a = kGroup; a < nw; a++) ok = mask_insert(s_tab[gi], &s_cnt[gi], m.ring[a * N + w]) && ok;
+                }
+            }
+        }
+        if (!ok) s_ovf[gi] = 1;
+    }
+    __syncthreads();
     unsigned long long w0 = 0ull, w1 = 0ull;
     if (active) {
-        const int n = meta_valence(m.meta[v]);
-        const RingRegs R = load_ring(m, v, n);
+        const bool set = use_set && !s_ovf[gi];
         const int2 vr = nl.vcand[v];
         const int cnt = min(vr.y - vr.x, 128);
         for (int i = lane; i < cnt; i += kGroup) {
             const int j = nl.cand[vr.x + i];
-            if (excluded_fast(m, R, v, j, nl.exclusion)) {
+            if (set ? mask_member(s_tab[gi], j) : excluded_fast(m, R, v, j, excl)) {
                 if (i < 64) w0 |= 1ull << i;
                 else w1 |= 1ull << (i - 64);
             }
