@@ -380,7 +380,7 @@ __device__ __forceinline__ bool coop_excluded(const TP& t, const CoopShared& S, 
 // two vertices opposite the edge (patch vertices 1 and 3), and nothing more
 // is evaluated when it is false.  owner (same topology): S.lost = some edge
 // of the parallel sweep's patch, the spokes h >> 1 and rims next(h) >> 1 of
-// every outgoing halfedge h of the four vertices (for_patch_edges_of), is
+// every outgoing halfedge h of the four vertices (the flip patch), is
 // owned by another key, tested on phase 2's halfedges; nothing more is
 // evaluated then.
 template <class TP>
@@ -1171,23 +1171,10 @@ void launch_serial_windows(const FlipMeshView& fm, const NListView& nl, const EP
 // locks.  Every phase is its own kernel, so no phase reads topology that
 // another thread is rewriting; the result depends only on the seed.
 
-// Edges of the flip patch of edge e (flip_utils.cpp:43-68): for the two
-// endpoints' and the two opposite vertices' outgoing halfedges h, the edges
-// of h and of next(h).  patch_vertex(t, e, w) is the w-th of those four
-// vertices; this visits its part of the patch.
-template <class F>
-__device__ __forceinline__ void for_patch_edges_of(const Topo& t, int e, int w, F&& f)
-{
-    int vh = patch_vertex(t, e, w);
-    int h0 = t.vhe(vh);
-    int h = h0;
-    int guard = 0;
-    do {
-        f(h >> 1);
-        f(t.next(h) >> 1);
-        h = t.cw(h);
-    } while (h != h0 && ++guard < 4 * kMaxValence);
-}
+// The flip patch of edge e (flip_utils.cpp:43-68) is the set of edges of h
+// and of next(h) over the outgoing halfedges h of its four vertices (the two
+// endpoints and the two opposite vertices, patch_vertex(t, e, 0..3)); the
+// claim phase and coop_patch's ownership test visit exactly these.
 
 __device__ __forceinline__ int pflip_edge(uint64_t seed, int round, int B, int i, int ne)
 {
@@ -1199,12 +1186,13 @@ __device__ __forceinline__ unsigned long long pflip_key(int round, int B, int i)
     return ((unsigned long long)(round + 1) << 32) | (unsigned)(B - i);
 }
 
-// Eight lanes per proposal, two per patch vertex: for_patch_edges_of's walk
-// around the vertex, lane 0 of the pair clockwise from v_he and lane 1
+// Eight lanes per proposal, two per patch vertex, walking the vertex's
+// outgoing halfedges: lane 0 of the pair clockwise from v_he, lane 1
 // counter-clockwise (the inverse of cw, prev(h) ^ 1), each claiming its
-// halfedges' spokes and rims until they meet, so the chain of dependent
-// loads is half as long.  The same edges get the same atomicMax, and the
-// maximum does not depend on the order of the atomics.
+// halfedges' edges and next edges until they meet, so the chain of
+// dependent loads is half as long.  The patch's edges get the same
+// atomicMax as in a one-lane walk, and the maximum does not depend on the
+// order of the atomics.
 __device__ __forceinline__ void pflip_claim_body(const FlipMeshView& fm, uint64_t seed, int round, int B, int nprop,
                                                  unsigned long long* __restrict__ owner, int lb)
 {
