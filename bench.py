@@ -57,6 +57,7 @@ class Hip:
         self.h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
                                           ctypes.c_void_p]
         self.h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
 
     def copy_calibration(self, stream, nbytes=COPY_BYTES):
         """One device-to-device copy of known size (the runtime's copy
@@ -82,6 +83,24 @@ class Hip:
         t = ctypes.c_float()
         assert self.h.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
         return float(t.value)
+
+    def destroy(self, e):
+        self.h.hipEventDestroy(e)
+
+    def marked_pass1_us(self, L, run, k):
+        """Mean duration (us) of the pass-1 kernel over k calls of run(),
+        each a properties pass whose pass-1 kernel stamps a pair of events at
+        its own start and end (tm_mark_pass1)."""
+        ev = [(self.event(), self.event()) for _ in range(k)]
+        for a, b in ev:
+            assert L.tm_mark_pass1(a, b) == 0
+            run()
+        assert L.tm_sync() == 0
+        d = [self.elapsed_ms(a, b) * 1e3 for a, b in ev]
+        for a, b in ev:
+            self.destroy(a)
+            self.destroy(b)
+        return float(np.mean(d))
 
 
 def cpu_baseline(v, f, params, budget_s):
@@ -333,28 +352,43 @@ def combined_rates(precision, hip, stream, n_replicas=32, evals=20, seed=2026120
         R.eval()
     hip.record(e1, stream)
     ms_group = hip.elapsed_ms(e0, e1) / evals
-    # pass 2 of the group alone (the gradient over pass 1's outputs of the
-    # last evaluation), the dominant kernel at this size
-    p2_bytes = float(np.sum((32.0 if precision != "fp64" else 64.0) * nv + 12.0 * nf))
+    # the two long kernels of the group, each timed live: pass 2 alone (the
+    # gradient over pass 1's outputs of the last evaluation), back to back,
+    # and the pass-1 kernel between events stamped at its start and end;
+    # the roofline line is the longer one
+    wide = precision == "fp64"
+    p1_bytes = float(np.sum((40.0 if wide else 20.0) * nv + 12.0 * nf))
+    p2_bytes = float(np.sum((64.0 if wide else 32.0) * nv + 12.0 * nf))
     hip.record(e0, stream)
     for _ in range(evals):
         R.eval(properties=False, gradient=True)
     hip.record(e1, stream)
     capi.check(L.tm_sync())
     us_p2 = hip.elapsed_ms(e0, e1) / evals * 1e3
-    roof = {"kernel": "k_pass2_grouped", "bound": "hbm", "achieved": p2_bytes / us_p2 / 1e3,
-            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": p2_bytes / us_p2 / 1e3 / HBM_PEAK_GBPS,
-            "algorithmic_bytes": p2_bytes, "duration_us": us_p2}
+    us_p1 = hip.marked_pass1_us(L, lambda: R.eval(properties=True, gradient=False), evals)
+    R.eval()
+    capi.check(L.tm_sync())
+    sfx = "_grouped_fp64" if wide else "_grouped"
+    lines = [{"kernel": "k_pass1_apart" + sfx, "bound": "hbm", "achieved": p1_bytes / us_p1 / 1e3,
+              "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": p1_bytes / us_p1 / 1e3 / HBM_PEAK_GBPS,
+              "algorithmic_bytes": p1_bytes, "duration_us": us_p1},
+             {"kernel": "k_pass2" + sfx, "bound": "hbm", "achieved": p2_bytes / us_p2 / 1e3,
+              "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": p2_bytes / us_p2 / 1e3 / HBM_PEAK_GBPS,
+              "algorithmic_bytes": p2_bytes, "duration_us": us_p2}]
     gp = os.path.join(ROOT, "profiles", "pmc_energy_32xL7_grouped.json")
-    if os.path.exists(gp) and precision == "mixed" and n_replicas == 32:
-        k2 = json.load(open(gp))["kernels"].get("k_pass2_grouped", {})
-        roof["traffic"] = k2.get("hbm_bytes_per_dispatch")
-        roof["rocprof_median_us"] = k2.get("avg_us")
+    if os.path.exists(gp) and n_replicas == 32:
+        ks = json.load(open(gp))["kernels"]
+        for r in lines:
+            kk = ks.get(r["kernel"], {})
+            r["traffic"] = kk.get("hbm_bytes_per_dispatch")
+            r["rocprof_median_us"] = kk.get("avg_us")
+    roof = max(lines, key=lambda r: r["duration_us"])
+    other_roof = min(lines, key=lambda r: r["duration_us"])
     out = {"replicas": n_replicas, "vertices": int(nv.sum()), "group_evals_per_s": 1e3 / ms_group,
            "ms_per_group_eval": ms_group, "replica_evals_per_s": n_replicas * 1e3 / ms_group,
            "algorithmic_bytes_per_group_eval": eval_bytes, "hbm_gbps": eval_bytes / (ms_group * 1e-3) / 1e9,
            "hbm_frac": eval_bytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-           "roofline": roof,
+           "roofline": roof, "roofline_other_pass": other_roof,
            "tuning": {k: tune[k] for k in ("rep_threads", "rep_batch", "pass2_threads", "pair_cache")},
            "workload": "32 icosphere L7 meshes jittered 1% (own seeds), vfrac U[0.55,0.9], kappa_b "
                        "log-uniform [20,80], cell-list repulsion excl 2 rc=1.5l; one grouped energy+gradient "
@@ -528,13 +562,18 @@ VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4    # 1024 SIMDs, one wave64 VALU op per 4 cy
 
 
 L7_N, L7_F = 163842, 327680
-# (title, kernel of the roofline line, its algorithmic bytes per launch and
-# how they are made up, the launches of pass 1)
-SINGLE_PROFILE = ("energy + gradient passes, icosphere L7, mixed precision", "k_pass2",
-                  32.0 * L7_N + 12.0 * L7_F, f"32 N + 12 F, N = {L7_N}, F = {L7_F}", PASS1_KERNELS)
+# (title, the kernels a roofline line may be about with their algorithmic
+# bytes per launch and how they are made up, the launches of pass 1); the
+# summary's line is about the longer of them
+SINGLE_PROFILE = ("energy + gradient passes, icosphere L7, mixed precision",
+                  (("k_pass1_apart", 20.0 * L7_N + 12.0 * L7_F, f"20 N + 12 F, N = {L7_N}, F = {L7_F}"),
+                   ("k_pass2", 32.0 * L7_N + 12.0 * L7_F, f"32 N + 12 F, N = {L7_N}, F = {L7_F}")),
+                  PASS1_KERNELS)
 GROUP_PROFILE = ("energy + gradient passes, 32 icosphere L7 meshes in one group, mixed precision",
-                 "k_pass2_grouped", 32 * (32.0 * L7_N + 12.0 * L7_F),
-                 f"32 meshes x (32 N + 12 F), N = {L7_N}, F = {L7_F}",
+                 (("k_pass1_apart_grouped", 32 * (20.0 * L7_N + 12.0 * L7_F),
+                   f"32 meshes x (20 N + 12 F), N = {L7_N}, F = {L7_F}"),
+                  ("k_pass2_grouped", 32 * (32.0 * L7_N + 12.0 * L7_F),
+                   f"32 meshes x (32 N + 12 F), N = {L7_N}, F = {L7_F}")),
                  tuple(k + "_grouped" for k in PASS1_KERNELS))
 
 
@@ -590,7 +629,7 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
     out = {"command": command, "copy_bytes": COPY_BYTES, "copy_factor_fetch": f_fetch,
            "copy_factor_write": f_write, "kernels": kernels, "durations": stats}
     json.dump(out, open(out_json, "w"), indent=1)
-    title, rk, rbytes, rtext, pass1 = what or SINGLE_PROFILE
+    title, roofs, pass1 = what or SINGLE_PROFILE
     lines = ["# rocprofv3 summary: " + title, "",
              "Commands (run from the repository root on one MI355X):", "", "```"] + command + ["```", "",
              f"Copy calibration: {COPY_BYTES} B device-to-device copy reads FETCH_SIZE x {f_fetch:.4f} "
@@ -603,10 +642,12 @@ def summarize(trace_dir, fetch_dir, write_dir, out_json, out_md, command, sq_dir
         lines.append(f"| {k} | {stats[k]['calls']} | {stats[k]['avg_us']:.2f} | "
                      f"{kk.get('fetch_kib', float('nan')):.1f} | {kk.get('write_kib', float('nan')):.1f} | "
                      f"{kk.get('hbm_bytes_per_dispatch', float('nan')):.0f} |")
-    if rk in stats:
-        # mixed pass 2 per vertex: positions 12, a and c 8, gradient 12 B; 12 B per face
+    # mixed precision per vertex: positions 12, a and c 8, gradient 12 B; 12 B per face
+    have = sorted((r for r in roofs if r[0] in stats), key=lambda r: -stats[r[0]]["avg_us"])
+    for i, (rk, rbytes, rtext) in enumerate(have):
         us = stats[rk]["avg_us"]
-        lines += ["", f"Roofline of {rk} (the longest kernel): {rbytes:.0f} algorithmic bytes per launch "
+        which = "the longest kernel" if i == 0 else "the other pass"
+        lines += ["", f"Roofline of {rk} ({which}): {rbytes:.0f} algorithmic bytes per launch "
                       f"({rtext}) / {us:.2f} us = {rbytes / us / 1e3:.0f} GB/s, "
                       f"{rbytes / us / 1e3 / HBM_PEAK_GBPS:.3f} of the 8 TB/s peak."]
     p1 = [k for k in pass1 if k in stats]
@@ -817,6 +858,9 @@ def main():
 
     p1 = np.array([per_launch_ms(capi.TM_EVAL_PROPS, ks)])
     p2 = np.array([per_launch_ms(capi.TM_EVAL_GRAD, ks)])
+    # the pass-1 kernel alone (k_pass1_apart; the first of the properties
+    # pass's launches), between events stamped at its start and end
+    us_k1 = hip.marked_pass1_us(L, lambda: L.tm_energy_eval_device(em, mesh, capi.TM_EVAL_PROPS), ks)
     capi.check(L.tm_energy_eval_device(em, mesh, both))
     capi.check(L.tm_sync())
 
@@ -855,18 +899,30 @@ def main():
     b_pass1 = wp * n + 12.0 * nf + wq * n
     b_pass2 = wp * n + 12.0 * nf + wq * n + wg * n
     t1, t2 = float(np.mean(p1)) * 1e-3, float(np.mean(p2)) * 1e-3
-    achieved = b_pass2 / t2 / 1e9
     eval_bytes = b_pass1 + b_pass2
-    traffic, valu_frac, prof_us = None, None, None
-    if os.path.exists(PROFILE_JSON) and args.level == 7 and args.precision == "mixed":
-        prof = json.load(open(PROFILE_JSON))
-        k2 = prof["kernels"].get("k_pass2")
-        if k2 is not None:
-            traffic = k2["hbm_bytes_per_dispatch"]
-            prof_us = k2.get("avg_us")
-            if "valu_issue_us" in k2:
+    # the roofline line is the longer of the two long kernels: the pass-1
+    # kernel (pass 1's bytes) and the pass-2 kernel (pass 2's bytes)
+    sfx = "_fp64" if args.precision == "fp64" else ""
+    prof = {}
+    if os.path.exists(PROFILE_JSON) and args.level == 7:
+        prof = json.load(open(PROFILE_JSON))["kernels"]
+
+    def roof_line(kernel, nbytes, us):
+        r = {"kernel": kernel, "bound": "hbm", "achieved": nbytes / us / 1e3, "peak": HBM_PEAK_GBPS,
+             "unit": "GB/s", "frac": nbytes / us / 1e3 / HBM_PEAK_GBPS, "traffic": None,
+             "algorithmic_bytes": nbytes, "duration_us": us, "rocprof_avg_us": None, "valu_issue_frac": None}
+        kk = prof.get(kernel)
+        if kk is not None:
+            r["traffic"] = kk["hbm_bytes_per_dispatch"]
+            r["rocprof_avg_us"] = kk.get("avg_us")
+            if "valu_issue_us" in kk:
                 # the time the kernel's VALU instructions take to issue / its duration
-                valu_frac = k2["valu_issue_us"] / (t2 * 1e6)
+                r["valu_issue_frac"] = kk["valu_issue_us"] / us
+        return r
+
+    roofs = [roof_line("k_pass1_apart" + sfx, b_pass1, us_k1), roof_line("k_pass2" + sfx, b_pass2, t2 * 1e6)]
+    roof = max(roofs, key=lambda r: r["duration_us"])
+    other_roof = min(roofs, key=lambda r: r["duration_us"])
     out = {
         "metric": "energy+grad evals/s",
         "metric_full": METRIC_FULL,
@@ -887,11 +943,9 @@ def main():
                    "precision": args.precision, "vertices": n, "faces": nf},
         "hbm_gbps_per_eval": eval_bytes * evals_per_s / 1e9,
         "algorithmic_bytes_per_eval": eval_bytes,
-        "kernels_ms": {"pass1_phase": t1 * 1e3, "k_pass2": t2 * 1e3},
-        "roofline": {"kernel": "k_pass2", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "algorithmic_bytes": b_pass2, "duration_us": t2 * 1e6, "rocprof_avg_us": prof_us,
-                     "valu_issue_frac": valu_frac},
+        "kernels_ms": {"pass1_phase": t1 * 1e3, "k_pass1_apart" + sfx: us_k1 * 1e-3, "k_pass2" + sfx: t2 * 1e3},
+        "roofline": roof,
+        "roofline_other_pass": other_roof,
         "pass1_phase": {"kernels": list(PASS1_KERNELS), "duration_us": t1 * 1e6, "algorithmic_bytes": b_pass1,
                         "achieved_gbps": b_pass1 / t1 / 1e9, "frac": b_pass1 / t1 / 1e9 / HBM_PEAK_GBPS},
         "wall_s_timed": wall,

@@ -159,6 +159,15 @@ const double* tm_energy_device_globals(tm_energy* em);    /* props[6], energy */
 int tm_sync(void);
 /* the library's HIP stream (hipStream_t), for event timing by callers */
 int tm_stream(void** out);
+/* timing aid (no counterpart in the reference; for callers that time one
+ * kernel of a launch sequence with HIP events): the next properties pass
+ * (tm_energy_eval_device or tm_replicas_eval with TM_EVAL_PROPS, or any
+ * single-mesh call that runs one) stamps the events `before` and `after`
+ * (hipEvent_t; either may be null) at the start and the end of its pass-1
+ * kernel (one launch over the whole group for tm_replicas_eval;
+ * hipExtLaunchKernel), then forgets them.
+ * The pass does the same work with or without them. */
+int tm_mark_pass1(void* before, void* after);
 
 /* ---- neighbour lists (nlist.h:15-24, nlist.cpp:10-53, bindings.cpp:893-985) ---- */
 /* make_nlist(mesh, eparams) */

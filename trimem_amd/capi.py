@@ -85,6 +85,7 @@ def lib():
         "tm_energy_set_tuning": (I, [P, ctypes.POINTER(tm_tuning)]),
         "tm_sync": (I, []),
         "tm_stream": (I, [ctypes.POINTER(P)]),
+        "tm_mark_pass1": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

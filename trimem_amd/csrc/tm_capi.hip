@@ -75,6 +75,7 @@ struct Runtime {
     int device = 0;
     hipStream_t stream = nullptr;
     void* scratch[kMaxDevices] = {};    // device address of `scratch` on each device
+    hipEvent_t pass1_marks[2] = {};     // tm_mark_pass1: taken by the next properties pass
 };
 Runtime g_rt;
 
@@ -756,7 +757,9 @@ void run_pass1(tm_energy& E, tm_mesh& M, int mixed, double* vprops)
     }
     ensure_masks(*E.nlist, M);
     const PassBuffers b = buffers(E, M);
-    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, b, vprops, g_rt.stream, E.tune);
+    hipEvent_t marks[2] = {g_rt.pass1_marks[0], g_rt.pass1_marks[1]};
+    g_rt.pass1_marks[0] = g_rt.pass1_marks[1] = nullptr;
+    launch_pass1(M.view(), E.nlist->view(), eparams_of(E), mixed, b, vprops, g_rt.stream, E.tune, marks);
     check_launch();
     if (b.pc.x_snap) E.nlist->pc_ok = true;    // recorded by this launch, or still valid
 }
@@ -1153,6 +1156,15 @@ int tm_stream(void** out)
     API_BEGIN
     ensure_runtime();
     *out = (void*)g_rt.stream;
+    API_END
+}
+
+int tm_mark_pass1(void* before, void* after)
+{
+    API_BEGIN
+    ensure_runtime();
+    g_rt.pass1_marks[0] = (hipEvent_t)before;
+    g_rt.pass1_marks[1] = (hipEvent_t)after;
     API_END
 }
 
@@ -1828,7 +1840,11 @@ int tm_replicas_eval(tm_replicas* rs, int flags)
 {
     API_BEGIN
     replicas_masks(*rs);
-    if (flags & TM_EVAL_PROPS) launch_pass1_grouped(rs->group(), g_rt.stream);
+    if (flags & TM_EVAL_PROPS) {
+        hipEvent_t marks[2] = {g_rt.pass1_marks[0], g_rt.pass1_marks[1]};
+        g_rt.pass1_marks[0] = g_rt.pass1_marks[1] = nullptr;
+        launch_pass1_grouped(rs->group(), g_rt.stream, marks);
+    }
     if (flags & TM_EVAL_GRAD) launch_pass2_grouped(rs->group(), g_rt.stream);
     check_launch();
     API_END

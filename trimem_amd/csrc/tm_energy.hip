@@ -27,6 +27,7 @@
 // Boundary vertices (ring with a faceless slot) take the OpenMesh-semantics
 // walker in FP64 in both paths, which reproduces the reference's boundary
 // branches of calc_sector_angle / calc_dihedral_angle.
+#include <hip/hip_ext.h>
 #include "tm_internal.h"
 #include "tm_om.h"
 
@@ -1690,19 +1691,30 @@ static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs
 // run first and pass 1 reads their rep_v; the cached query runs after pass
 // 1, which measures the displacements its replay decision needs, and adds
 // column 5 of the partials itself.
-void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s)
+// A launch that, given marks, stamps marks[0] and marks[1] (either may be
+// null) at the kernel's own start and end (hipExtLaunchKernel), so a caller
+// can time one kernel of a launch sequence without an event packet between
+// launches.
+template <class F, class... Args>
+void launch_marked(F kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* marks, Args... args)
+{
+    if (marks && (marks[0] || marks[1])) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, marks[0], marks[1], 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
+void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s, const hipEvent_t* marks)
 {
     if (a.m.nv == 0) return;
     const dim3 grid(a.nblocks), block(kPass1Verts);
     if (a.rep_v && a.pc.x_snap && !a.pc.force) {
-        if (mixed) hipLaunchKernelGGL(k_pass1_apart<float>, grid, block, 0, s, a);
-        else hipLaunchKernelGGL(k_pass1_apart<double>, grid, block, 0, s, a);
+        if (mixed) launch_marked(k_pass1_apart<float>, grid, block, s, marks, a);
+        else launch_marked(k_pass1_apart<double>, grid, block, s, marks, a);
         launch_rep(a, t, s);
         return;
     }
     launch_rep(a, t, s);
-    if (mixed) hipLaunchKernelGGL(k_pass1<float>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(k_pass1<double>, grid, block, 0, s, a);
+    if (mixed) launch_marked(k_pass1<float>, grid, block, s, marks, a);
+    else launch_marked(k_pass1<double>, grid, block, s, marks, a);
 }
 
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s)
@@ -1738,13 +1750,13 @@ void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStrea
 }
 
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* vprops, hipStream_t s, const Tuning& t)
+                  double* vprops, hipStream_t s, const Tuning& t, const hipEvent_t* marks)
 {
     const PassArgs a = pass_args(m, nl, p, b, vprops, nullptr);
     if (b.nlist > 0)
         hipLaunchKernelGGL(k_generic_props, dim3(nblocks(b.nlist, 128)), dim3(128), 0, s, m, nl, p, b.list, b.nlist,
                            b.gprops);
-    launch_pass1_body(a, mixed, t, s);
+    launch_pass1_body(a, mixed, t, s, marks);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, a);
 }
 
@@ -1776,7 +1788,7 @@ void launch_rep_grouped(const GroupLaunch& g, hipStream_t s)
     rep_dispatch(g.tune, true, g.h_args[0], &g, s);
 }
 
-void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
+void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s, const hipEvent_t* marks)
 {
     for (int r = 0; r < g.R; r++)
         if (g.h_args[r].nlist > 0)
@@ -1785,13 +1797,13 @@ void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s)
     const dim3 grid(g.total_p1), block(kPass1Verts);
     if (g.total_rc > 0) {
         // the cached query after pass 1, as for single meshes
-        if (g.mixed) hipLaunchKernelGGL(k_pass1_apart_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-        else hipLaunchKernelGGL(k_pass1_apart_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        if (g.mixed) launch_marked(k_pass1_apart_grouped<float>, grid, block, s, marks, g.d_args, g.d_block0_p1, g.R);
+        else launch_marked(k_pass1_apart_grouped<double>, grid, block, s, marks, g.d_args, g.d_block0_p1, g.R);
         launch_rep_grouped(g, s);
     } else {
         launch_rep_grouped(g, s);
-        if (g.mixed) hipLaunchKernelGGL(k_pass1_grouped<float>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
-        else hipLaunchKernelGGL(k_pass1_grouped<double>, grid, block, 0, s, g.d_args, g.d_block0_p1, g.R);
+        if (g.mixed) launch_marked(k_pass1_grouped<float>, grid, block, s, marks, g.d_args, g.d_block0_p1, g.R);
+        else launch_marked(k_pass1_grouped<double>, grid, block, s, marks, g.d_args, g.d_block0_p1, g.R);
     }
     hipLaunchKernelGGL(k_finalize_grouped, dim3(g.R), dim3(kFinThreads), 0, s, g.d_args);
 }

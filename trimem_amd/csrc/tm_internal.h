@@ -105,8 +105,11 @@ int tuned_blocks(int nv, int threads);    // blocks of 8-lane groups (the query)
 int pass2_blocks(int nv, int threads);    // blocks of one lane per vertex (pass 2), a multiple of 8
 // the repulsion query and the pass-2 kernel alone (for timing)
 void launch_rep(const PassArgs& a, const Tuning& t, hipStream_t s);
-// the repulsion query and the pass-1 kernel (fused or apart), no finalize
-void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s);
+// the repulsion query and the pass-1 kernel (fused or apart), no finalize;
+// marks (optional): HIP events stamped at the pass-1 kernel's start and end
+// (either may be null)
+void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s,
+                       const hipEvent_t* marks = nullptr);
 void launch_pass2_kernel(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s);
 // R replicas: d_args in device memory, h_args its host copy (for the
 // launches of the boundary kernels), one prefix table of blocks per kernel
@@ -128,11 +131,11 @@ struct GroupLaunch {
     const int* d_block0_rc;
     int total_rc;
 };
-void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s);
+void launch_pass1_grouped(const GroupLaunch& g, hipStream_t s, const hipEvent_t* marks = nullptr);
 void launch_pass2_grouped(const GroupLaunch& g, hipStream_t s);
 void launch_rep_grouped(const GroupLaunch& g, hipStream_t s);
 void launch_pass1(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
-                  double* vprops, hipStream_t s, const Tuning& t = Tuning());
+                  double* vprops, hipStream_t s, const Tuning& t = Tuning(), const hipEvent_t* marks = nullptr);
 void launch_pass2(const MeshView& m, const NListView& nl, const EParams& p, int mixed, const PassBuffers& b,
                   double* grad, hipStream_t s, const Tuning& t = Tuning());
 void launch_halfedge_geometry(const MeshView& m, int h, double* out, hipStream_t s);
