@@ -790,7 +790,6 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     const MeshView& m = A.m;
     const EParams& p = A.p;
     __shared__ double s_red[(kPass1Verts / 64) * 6];
-    __shared__ double s_out[6];
     const int v = lb * kPass1Verts + threadIdx.x;
     const bool active = v < m.nv;
     const int vc = active ? v : m.nv - 1;    // lanes past the last vertex load a valid one
@@ -828,17 +827,33 @@ __device__ __forceinline__ void pass1_body(const PassArgs& A, int lb)
     }
     if (A.vprops && active)
         for (int k = 0; k < (kApart ? 5 : 6); k++) A.vprops[6 * v + k] = pr[k];
+    // the block's partial sums by block_sum6's tree (the cached query sums
+    // column 5 of the same vertices by that tree) and, with the cache, its
+    // largest displacement, in one round through LDS
+    constexpr int kWaves = kPass1Verts / 64;
+    static_assert(kWaves == 4, "the displacement maximum below reads four waves");
+    __shared__ double s_disp[kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double a = pr[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) s_red[wave * 6 + k] = a;
+    }
     if (kApart && A.pc.x_snap) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) disp = fmax(disp, __shfl_xor(disp, off, 64));
-        __shared__ double s_disp[kPass1Verts / 64];
-        if ((threadIdx.x & 63) == 0) s_disp[threadIdx.x >> 6] = disp;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            A.pc.disp_part[lb] = fmax(fmax(s_disp[0], s_disp[1]), fmax(s_disp[2], s_disp[3]));
+        if (lane == 0) s_disp[wave] = disp;
     }
-    block_sum6(pr, s_red, s_out);
-    if (threadIdx.x < (kApart ? 5 : 6)) A.partials[threadIdx.x * A.nblocks + lb] = s_out[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < (kApart ? 5 : 6)) {
+        double a = 0;
+        for (int w = 0; w < kWaves; w++) a += s_red[w * 6 + threadIdx.x];
+        A.partials[threadIdx.x * A.nblocks + lb] = a;
+    }
+    if (kApart && A.pc.x_snap && threadIdx.x == 63)
+        A.pc.disp_part[lb] = fmax(fmax(s_disp[0], s_disp[1]), fmax(s_disp[2], s_disp[3]));
 }
 
 template <class T>
