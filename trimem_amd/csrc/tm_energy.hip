@@ -1713,8 +1713,10 @@ static void p2_dispatch(int mixed, const Tuning& t, bool grouped, const PassArgs
 template <class F, class... Args>
 void launch_marked(F kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* marks, Args... args)
 {
-    if (marks && (marks[0] || marks[1])) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, marks[0], marks[1], 0, args...);
-    else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+    if (marks && (marks[0] || marks[1]))
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, marks[0], marks[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
 void launch_pass1_body(const PassArgs& a, int mixed, const Tuning& t, hipStream_t s, const hipEvent_t* marks)

@@ -265,7 +265,7 @@ constexpr int kCoopLanes = 16;
 // patches than the GPU holds at once, and occupancy, not one patch's
 // latency, then bounds it (at 153 registers, 12 workgroups per CU, a round
 // over 32 meshes of 491520 edges took 21% longer).
-#define kCoopOccupancy __attribute__((amdgpu_waves_per_eu(4)))
+#define TM_COOP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 constexpr int kCoopChunk = 4 * kCoopLanes;
 
 struct CoopShared {
@@ -1266,11 +1266,10 @@ __device__ __forceinline__ void pflip_eval_body(const FlipMeshView& fm, const NL
     }
 }
 
-__global__ void __launch_bounds__(64) kCoopOccupancy k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed,
-                                                   int round, int B, int nprop,
-                                                   const unsigned long long* __restrict__ owner,
-                                                   int* __restrict__ status, double* __restrict__ dold,
-                                                   double* __restrict__ dnew)
+__global__ void __launch_bounds__(64) TM_COOP_OCCUPANCY
+k_pflip_eval(FlipMeshView fm, NListView nl, EParams p, uint64_t seed, int round, int B, int nprop,
+             const unsigned long long* __restrict__ owner, int* __restrict__ status, double* __restrict__ dold,
+             double* __restrict__ dnew)
 {
     pflip_eval_body(fm, nl, p, seed, round, B, nprop, owner, status, dold, dnew, blockIdx.x);
 }
@@ -1402,8 +1401,8 @@ __global__ void __launch_bounds__(64) k_pflip_claim_grouped(const PflipArgs* __r
     pflip_claim_body(a.fm, a.seed, round, a.batch, nprop, a.owner, blockIdx.x - block0[r]);
 }
 
-__global__ void __launch_bounds__(64) kCoopOccupancy k_pflip_eval_grouped(const PflipArgs* __restrict__ A,
-                                                           const int* __restrict__ block0, int R, int round)
+__global__ void __launch_bounds__(64) TM_COOP_OCCUPANCY
+k_pflip_eval_grouped(const PflipArgs* __restrict__ A, const int* __restrict__ block0, int R, int round)
 {
     const int r = find_replica(block0, R, blockIdx.x);
     const PflipArgs& a = A[r];
@@ -1432,16 +1431,16 @@ __global__ void k_pflip_apply_grouped(const PflipArgs* __restrict__ A, const int
 }
 
 void launch_flip_parallel(const FlipMeshView& fm, const NListView& nl, const EParams& p, int nflips, int batch,
-                          int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status, double* dprops,
-                          double* state, hipStream_t s)
+                          int window, uint64_t seed, const Globals* g0, unsigned long long* owner, int* status,
+                          double* dprops, double* state, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pflip_init, dim3(1), dim3(64), 0, s, g0, p, state);
     const int rounds = (nflips + batch - 1) / batch;
     for (int r = 0; r < rounds; r++) {
         int nprop = std::min(batch, nflips - r * batch);
         dim3 grid((nprop + 127) / 128), block(128);
-        hipLaunchKernelGGL(k_pflip_claim, dim3((nprop + kClaimPerBlock - 1) / kClaimPerBlock), dim3(64), 0, s, fm, seed, r,
-                           batch, nprop, owner);
+        const dim3 claim_grid((nprop + kClaimPerBlock - 1) / kClaimPerBlock);
+        hipLaunchKernelGGL(k_pflip_claim, claim_grid, dim3(64), 0, s, fm, seed, r, batch, nprop, owner);
         hipLaunchKernelGGL(k_pflip_eval, dim3(nprop, 2), dim3(64), 0, s, fm, nl, p, seed, r, batch, nprop, owner,
                            status, dprops, dprops + 6 * (size_t)batch);
         hipLaunchKernelGGL(k_pflip_scan, dim3(1), dim3(kScanThreads), 0, s, p, seed, r, batch, nprop,
