@@ -194,6 +194,14 @@ int tm_flip_serial(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t see
  * patches) flip in parallel and a sequential FP64 Metropolis scan in
  * proposal order accepts or rejects them.  flip_ratio > 1 -> TM_E_RANGE. */
 int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t seed, int batch, int* accepted);
+/* the last parallel sweep's own bookkeeping (tm_flip_parallel or
+ * tm_replicas_pflip on this energy manager; no counterpart in the
+ * reference): the running global properties (area, volume, curvature,
+ * bending, tethering, repulsion), the energy and the accepted count, as the
+ * Metropolis scan left them.  The scan updates them by each accepted flip's
+ * patch difference, so they equal the properties of the final topology when
+ * no two flips accepted in one round shared a patch edge. */
+int tm_energy_pflip_state(tm_energy* em, double* out /* 8 */);
 /* flip_serial with caller-supplied proposals (edge ids) and uniforms; a
  * uniform is consumed only by proposals that pass is_flip_ok */
 int tm_flip_serial_injected(tm_energy* em, tm_mesh* mesh, int n_proposals, const int* proposals,

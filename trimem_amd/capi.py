@@ -86,6 +86,7 @@ def lib():
         "tm_sync": (I, []),
         "tm_stream": (I, [ctypes.POINTER(P)]),
         "tm_mark_pass1": (I, [P, P]),
+        "tm_energy_pflip_state": (I, [P, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
