@@ -200,7 +200,8 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
  * bending, tethering, repulsion), the energy and the accepted count, as the
  * Metropolis scan left them.  The scan updates them by each accepted flip's
  * patch difference, so they equal the properties of the final topology when
- * no two flips accepted in one round shared a patch edge. */
+ * no two flips accepted in one round shared a patch edge.  A sweep with no
+ * proposals leaves them as they were; all zero before the first sweep. */
 int tm_energy_pflip_state(tm_energy* em, double* out /* 8 */);
 /* flip_serial with caller-supplied proposals (edge ids) and uniforms; a
  * uniform is consumed only by proposals that pass is_flip_ok */

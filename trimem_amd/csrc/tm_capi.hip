@@ -1386,7 +1386,7 @@ int tm_flip_parallel(tm_energy* em, tm_mesh* mesh, double flip_ratio, uint64_t s
     sync();
     mesh->host_stale = true;
     if (st[7] > 0 || ovf) mesh->topo_version++;
-    std::copy(st, st + 8, em->pf_last);
+    if (nflips > 0) std::copy(st, st + 8, em->pf_last);
     if (ovf) throw TmError(TM_E_UNSUPPORTED, "vertex valence exceeds the ring-table width (32)");
     if (accepted) *accepted = (int)st[7];
     API_END
@@ -1839,7 +1839,7 @@ int tm_replicas_pflip(tm_replicas* rs, double flip_ratio, const uint64_t* seeds,
         tm_mesh& M = *S.meshes[r];
         M.host_stale = true;
         if (st[8 * (size_t)r + 7] > 0 || ovf[r]) M.topo_version++;
-        std::copy(&st[8 * (size_t)r], &st[8 * (size_t)r] + 8, S.ems[r]->pf_last);
+        if (h[r].nflips > 0) std::copy(&st[8 * (size_t)r], &st[8 * (size_t)r] + 8, S.ems[r]->pf_last);
         overflow = overflow || ovf[r];
         if (accepted) accepted[r] = (int)st[8 * (size_t)r + 7];
     }
