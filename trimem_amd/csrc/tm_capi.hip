@@ -9,6 +9,7 @@
 #include "tm_internal.h"
 #include "tm_surrogate.h"
 
+#include <cfloat>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -429,8 +430,7 @@ struct tm_nlist {
     DevBuf<int> dm_counts, dm_off;
     DevBuf<double> dm_d;
     DevBuf<int> dm_i, dm_j;
-    DevBuf<double> box, box_part, pd_dist;
-    DevBuf<unsigned> box_counter;
+    DevBuf<double> box_part, pd_dist;
     DevBuf<int> pd_j, pd_count;
     int nv = 0;
     NListView view() const { return v; }
@@ -468,17 +468,19 @@ void build_nlist(tm_nlist& L, tm_mesh& M)
     if (!(L.rlist > 0) || !std::isfinite(L.rlist)) throw TmError(TM_E_INVALID, "rlist must be positive");
     const int n = M.nv;
     L.nv = n;
-    L.box.alloc(6);
-    L.box_part.alloc(box_scratch_doubles());
-    if (!L.box_counter.p) {
-        L.box_counter.alloc(1);
-        HIPCHECK(hipMemsetAsync(L.box_counter.p, 0, sizeof(unsigned), g_rt.stream));
-    }
-    launch_box(M.x.p, n, L.box_part.p, L.box_counter.p, L.box.p, g_rt.stream);
+    const int nbox = box_scratch_doubles();
+    L.box_part.alloc(nbox);
+    launch_box(M.x.p, n, L.box_part.p, g_rt.stream);
     check_launch();
-    double tight[6];    // the vertices' own bounding box
-    d2h(tight, L.box.p, sizeof(tight));
+    std::vector<double> part(nbox);
+    d2h(part.data(), L.box_part.p, sizeof(double) * nbox);
     sync();
+    double tight[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};    // the vertices' own box
+    for (int b = 0; b < nbox / 6; b++)
+        for (int k = 0; k < 3; k++) {
+            tight[k] = std::fmin(tight[k], part[6 * b + k]);
+            tight[3 + k] = std::fmax(tight[3 + k], part[6 * b + 3 + k]);
+        }
     // the reference's box (cell_list.h:46-56): its bounds start at min =
     // (+max, 0, 0) and max = (-1, 0, 0), so it spans 0 in y and z
     double box[6];

@@ -153,7 +153,7 @@ void launch_build_ring(const MeshView& m, int* ring, int* meta, int* overflow, h
 // neighbour lists (tm_nlist.hip)
 int box_scratch_doubles();
 void launch_fill_ints(int* a, int na, int va, int* b, int nb, int vb, int* c, int nc, int vc, hipStream_t s);
-void launch_box(const double* x, int n, double* scratch, unsigned* counter, double* box6, hipStream_t s);
+void launch_box(const double* x, int n, double* part, hipStream_t s);
 void launch_cell_ids(const double* x, int n, const NListView& nl, int* vert_cell, int* counts, hipStream_t s);
 void launch_exclusive_scan(const int* in, int* out, int n, int* block_sums, hipStream_t s);
 int scan_block_sums_size(int n);

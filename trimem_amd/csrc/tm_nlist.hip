@@ -30,24 +30,38 @@
 namespace tmh {
 
 // ----------------------------------------------------------------- box ----
-// Bounding box of the vertices: min = (+DBL_MAX)^3 and max = (-DBL_MAX)^3
-// for no vertex.  The host applies the reference's initial values
-// (cell_list.h:46-47) to it.  Grid-stride partial min/max per block, and the
-// last block to finish combines the partials.
+// Bounding box of the vertices, as kBoxBlocks per-block partial minima and
+// maxima (part[6 b + k], min in k < 3, max in k >= 3; +DBL_MAX and -DBL_MAX
+// for a block without vertices).  The host, which reads the box back anyway
+// to size the grid, combines them (min and max are exact in any order) and
+// applies the reference's initial values (cell_list.h:46-47); a last-block
+// combine would need a device-scope fence in every block, which costs more
+// than the whole pass.
 constexpr int kBoxBlocks = 64;
-__global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n, double* __restrict__ part,
-                                              unsigned* __restrict__ counter, double* __restrict__ box)
+__global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n, double* __restrict__ part)
 {
     __shared__ double s[6][4];
-    __shared__ int s_last;
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        for (int k = 0; k < 3; k++) {
-            double v = x[3 * i + k];
-            if (v < mn[k]) mn[k] = v;
-            if (v > mx[k]) mx[k] = v;
+    // kBoxBatch points per thread and step, all their loads in flight before
+    // the first comparison
+    constexpr int kBoxBatch = 8;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kBoxBatch * stride) {
+        double v[kBoxBatch][3];
+#pragma unroll
+        for (int u = 0; u < kBoxBatch; u++) {
+            const int i = i0 + u * stride;
+#pragma unroll
+            for (int k = 0; k < 3; k++) v[u][k] = i < n ? x[3 * i + k] : __builtin_nan("");    // neither test takes NaN
         }
+#pragma unroll
+        for (int u = 0; u < kBoxBatch; u++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (v[u][k] < mn[k]) mn[k] = v[u][k];
+                if (v[u][k] > mx[k]) mx[k] = v[u][k];
+            }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; k++) {
@@ -69,47 +83,9 @@ __global__ void __launch_bounds__(256) k_box(const double* __restrict__ x, int n
         for (int w = 1; w < 4; w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
         part[blockIdx.x * 6 + k] = r;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // the last block combines the blocks' partials, all of them in flight at
-    // once (min and max are exact in any order)
-    for (int k = 0; k < 3; k++) {
-        mn[k] = DBL_MAX;
-        mx[k] = -DBL_MAX;
-    }
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-        for (int k = 0; k < 3; k++) {
-            mn[k] = fmin(mn[k], part[b * 6 + k]);
-            mx[k] = fmax(mx[k], part[b * 6 + 3 + k]);
-        }
-    for (int k = 0; k < 3; k++) {
-        for (int off = 32; off > 0; off >>= 1) {
-            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
-            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
-        }
-    }
-    __syncthreads();    // s was last read before the counter barrier; reuse it
-    if (lane == 0) {
-        for (int k = 0; k < 3; k++) {
-            s[k][wave] = mn[k];
-            s[3 + k][wave] = mx[k];
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        int k = threadIdx.x;
-        double r = s[k][0];
-        for (int w = 1; w < 4; w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
-        box[k] = r;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
 }
 
-int box_scratch_doubles() { return kBoxBlocks * 6 + 8; }
+int box_scratch_doubles() { return kBoxBlocks * 6; }
 
 // Up to three int ranges set to a value each, in one launch (the list build
 // issues these fills back to back after its one host round trip).
@@ -128,9 +104,9 @@ void launch_fill_ints(int* a, int na, int va, int* b, int nb, int vb, int* c, in
     hipLaunchKernelGGL(k_fill_ints, dim3((n + 255) / 256), dim3(256), 0, s, a, na, va, b, nb, vb, c, nc, vc);
 }
 
-void launch_box(const double* x, int n, double* scratch, unsigned* counter, double* box6, hipStream_t s)
+void launch_box(const double* x, int n, double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_box, dim3(kBoxBlocks), dim3(256), 0, s, x, n, scratch, counter, box6);
+    hipLaunchKernelGGL(k_box, dim3(kBoxBlocks), dim3(256), 0, s, x, n, part);
 }
 
 // ------------------------------------------------------------- binning ----
